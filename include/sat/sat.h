@@ -1,0 +1,203 @@
+/*
+ * sat.h - C ABI of the MI355X (gfx950) summed-area-table library.
+ *
+ * Every entry point takes plain device pointers, element pitches and sizes,
+ * returns a sat_status, never aborts and never frees caller memory. Streams
+ * are passed as `void*` (a hipStream_t; NULL selects the default stream).
+ * All work is stream-ordered; nothing here synchronises the host unless its
+ * comment says so.
+ *
+ * Reference interfaces replaced (pengdada/SAT, paths relative to its root):
+ *   BlockScan::blockScan       integralImage/ScanRow-BRLT.cu:24-25
+ *   SerielScan::serielScan     integralImage/BRLT-Scan.cu:14-15
+ *   IncrementScan::ScanX       integralImage/ScanRowColumn.cu:144-145
+ *   IncrementScan::ScanY       integralImage/ScanRowColumn.cu:15-16
+ *   <NS>::Test<TSrc,TDst,SCAN> two-pass full SAT compositions
+ *                              ScanRow-BRLT.cu:225-226, BRLT-Scan.cu:237-238,
+ *                              ScanRowColumn.cu:475-476
+ *   DataType / SCAN enums      integralImage/integralImage.h:22-37
+ *   algorithm ids              integralImage/main.cpp:20-34
+ */
+#ifndef SAT_AMD_SAT_H
+#define SAT_AMD_SAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAT_ABI_VERSION 1
+
+typedef enum sat_status {
+  SAT_OK = 0,
+  SAT_ERR_NULL = 1,          /* a required pointer is NULL                         */
+  SAT_ERR_INVALID = 2,       /* bad size, pitch, count, box or algorithm id        */
+  SAT_ERR_DTYPE = 3,         /* dtype pair not supported by this entry point       */
+  SAT_ERR_WORKSPACE = 4,     /* caller workspace too small                         */
+  SAT_ERR_HIP = 5,           /* a HIP runtime call failed (launch, alloc, copy)    */
+  SAT_ERR_UNSUPPORTED = 6    /* valid request this build does not implement        */
+} sat_status;
+
+/* Numerically equal to the reference's DataType (integralImage.h:22-32). */
+typedef enum sat_dtype {
+  SAT_U8 = 0,
+  SAT_I8 = 1,
+  SAT_U16 = 2,
+  SAT_I16 = 3,
+  SAT_U32 = 4,
+  SAT_I32 = 5,
+  SAT_F32 = 6,
+  SAT_F64 = 7
+} sat_dtype;
+
+/* Numerically equal to the reference's SCAN (integralImage.h:34-37). Selects
+ * the in-wave scan order (Kogge-Stone or Ladner-Fischer); integer results do
+ * not depend on it, floating-point results may differ in rounding. */
+typedef enum sat_scan { SAT_SCAN_KOGGE_STONE = 0, SAT_SCAN_LF = 1 } sat_scan;
+
+/* 0..2 are the reference's algorithm ids (main.cpp:20-34); each composes two
+ * passes through a temporary like the reference's Test<> harnesses. 3 is this
+ * library's one-pass kernel (reads the input once, writes the output once). */
+typedef enum sat_alg {
+  SAT_ALG_SCANROW_BRLT = 0,   /* BlockScan:     blockScan(blockScan(A))        */
+  SAT_ALG_BRLT_SCANROW = 1,   /* SerielScan:    serielScan(serielScan(A))      */
+  SAT_ALG_SCANROW_COLUMN = 2, /* IncrementScan: ScanY(ScanX(A))                */
+  SAT_ALG_SINGLE_PASS = 3     /* one launch, decoupled look-back in 2-D        */
+} sat_alg;
+
+/* Supported (TSrc, TDst) pairs: the reference dispatchers' set
+ * (ScanRow-BRLT.cu:274-376): u8->{u32,f32,f64}, i8->{i32,f32,f64},
+ * i32->{i32,f64}, f32->{f32,f64}, f64->f64; plus u32->u32. Integer outputs
+ * wrap modulo 2^32. */
+int sat_dtype_pair_supported(int tin, int tout);
+const char* sat_status_string(int status);
+int sat_abi_version(void);
+/* Size in bytes of one element of `dtype`, 0 if unknown. */
+size_t sat_dtype_size(int dtype);
+
+/* ---------------------------------------------------------------------------
+ * Pass-level drop-ins. Pitches (`widthStride`, `heightStride`) are in elements
+ * of the pointed-to type, as GpuData::DataPitch() returns them
+ * (cudaLib.cuh:178-180). The tuning template parameters of the reference
+ * kernels (BLOCK_SIZE, SMEM_COUNT, BLOCK_DIM_*, COUNT_PER_THREAD, WARP_COUNT)
+ * have no counterpart: tiles are chosen for wave64 inside the library. These
+ * calls use a library-managed carry workspace per (device, stream), which is
+ * allocated (hipMalloc, once, growing) on first use.
+ * ------------------------------------------------------------------------- */
+
+/* BlockScan::blockScan (ScanRow-BRLT.cu:24-25): row scan, then
+ * block-register-local transpose; out[x*heightStride + y] = sum_{k<=x} in[y*widthStride + k].
+ * `in` is height x width, `out` is width x height. */
+int sat_block_scan(int scan, int tin, int tout, const void* in, void* out, uint32_t width,
+                   uint32_t widthStride, uint32_t height, uint32_t heightStride, void* stream);
+
+/* SerielScan::serielScan (BRLT-Scan.cu:14-15): block-register-local
+ * transpose, then serial row scan; same result layout as sat_block_scan. The
+ * reference kernel has no scan-type parameter (BRLT-Scan.cu:14). */
+int sat_seriel_scan(int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
+                    uint32_t height, uint32_t heightStride, void* stream);
+
+/* IncrementScan::ScanX (ScanRowColumn.cu:144-145): row scan in place layout,
+ * out[y*widthStride + x] = sum_{k<=x} in[y*widthStride + k]; one stride for
+ * both buffers, counted in elements of each buffer's own type. */
+int sat_scan_x(int scan, int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
+               uint32_t height, void* stream);
+
+/* IncrementScan::ScanY (ScanRowColumn.cu:15-16): column scan,
+ * out[y*widthStride + x] = sum_{k<=y} in[k*widthStride + x]. */
+int sat_scan_y(int scan, int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
+               uint32_t height, void* stream);
+
+/* Row / column scans with separate input and output pitches (elements). */
+int sat_scan_rows(int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                  uint32_t width, uint32_t height, void* stream);
+int sat_scan_cols(int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                  uint32_t width, uint32_t height, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Full summed-area table: out[y][x] = sum_{k<=y, l<=x} in[k][l]
+ * (IntegralImageSerial, integralImage.h:105-124). `in` and `out` are
+ * height x width, pitches in elements of their own types. `workspace` may be
+ * NULL (library-managed) or caller device memory of at least
+ * sat_integral_workspace_bytes() bytes, zero-filled once before its first
+ * use and not shared by calls that can overlap.
+ * ------------------------------------------------------------------------- */
+size_t sat_integral_workspace_bytes(int alg, int tin, int tout, uint32_t width, uint32_t height);
+int sat_integral(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out,
+                 int64_t out_pitch, uint32_t width, uint32_t height, void* workspace, size_t workspace_bytes,
+                 void* stream);
+
+/* Same over `planes` images of one shape, plane strides in elements (e.g. the
+ * N*C planes of an NCHW tensor). */
+size_t sat_integral_planes_workspace_bytes(int alg, int tin, int tout, uint32_t width, uint32_t height,
+                                           uint32_t planes);
+int sat_integral_planes(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch,
+                        int64_t in_plane_stride, void* out, int64_t out_pitch, int64_t out_plane_stride,
+                        uint32_t width, uint32_t height, uint32_t planes, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/* Reverse-direction SAT (the adjoint of sat_integral):
+ * out[y][x] = sum_{k>=y, l>=x} in[k][l]. One pass. tin == tout, f32 or f64
+ * (also u32 / i32, which wrap). No reference counterpart. */
+size_t sat_integral_reverse_workspace_bytes(int dtype, uint32_t width, uint32_t height);
+int sat_integral_reverse(int scan, int dtype, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                         uint32_t width, uint32_t height, void* workspace, size_t workspace_bytes, void* stream);
+
+size_t sat_integral_reverse_planes_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes);
+int sat_integral_reverse_planes(int scan, int dtype, const void* in, int64_t in_pitch, int64_t in_plane_stride,
+                                void* out, int64_t out_pitch, int64_t out_plane_stride, uint32_t width,
+                                uint32_t height, uint32_t planes, void* workspace, size_t workspace_bytes,
+                                void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched SAT of many images of uneven sizes in ONE persistent launch walking
+ * a queue of (image, tile) items. No reference counterpart.
+ * Images may mix dtype pairs whose outputs have the same width (all 4-byte
+ * or all 8-byte outputs).
+ * ------------------------------------------------------------------------- */
+typedef struct sat_image {
+  const void* in;
+  void* out;
+  int64_t in_pitch;  /* elements of tin  */
+  int64_t out_pitch; /* elements of tout */
+  uint32_t width;
+  uint32_t height;
+  int32_t tin;
+  int32_t tout;
+} sat_image;
+
+size_t sat_batch_workspace_bytes(const sat_image* images, int count);
+/* Validates the batch, builds the work queue on the host and copies it into
+ * the workspace (blocking copy; call outside graph capture). */
+int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_t workspace_bytes, void* stream);
+/* One persistent launch over the prepared queue. Allocates nothing, copies
+ * nothing: may be captured in a hipGraph and replayed. `ctas_per_cu` = 0
+ * picks the tuned / default residency. */
+int sat_batch_run(void* workspace, int ctas_per_cu, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Box-filter readout on summed-area tables (f32 or f64), P planes.
+ * boxes: P*nbox records of 4 int32 {y0, x0, y1, x1}, inclusive corners,
+ * 0 <= y0 <= y1 < height, 0 <= x0 <= x1 < width.
+ * out[p*nbox + b] = S(y1,x1) - S(y0-1,x1) - S(y1,x0-1) + S(y0-1,x0-1)
+ * (terms with a -1 index are 0). No reference counterpart.
+ * ------------------------------------------------------------------------- */
+int sat_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height,
+                 uint32_t planes, const int32_t* boxes, uint32_t nbox, void* out, void* stream);
+
+/* Gradient of sum_b g[p*nbox+b] * boxsum_b(SAT(x_p)) with respect to x:
+ * grad_x[p][y][x] = sum over boxes b of plane p containing (y, x) of g[p*nbox+b].
+ * Computed as the reverse SAT of the boxes' corner impulses without
+ * materialising the impulse image. grad_x is overwritten. */
+size_t sat_box_grad_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes, uint32_t nbox);
+int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
+                 int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAT_AMD_SAT_H */

@@ -1,0 +1,177 @@
+// Box-filter readout on summed-area tables and its fused gradient.
+#include "sat_launch.hpp"
+
+namespace satdev {
+
+// A box {y0, x0, y1, x1} (inclusive) is valid when it lies inside the image
+// and is not empty. Invalid boxes read nothing: their sum is NaN and they
+// contribute no gradient.
+__device__ __forceinline__ bool box_ok(int4 b, uint32_t W, uint32_t H) {
+  return b.x >= 0 && b.y >= 0 && b.x <= b.z && b.y <= b.w && (uint32_t)b.z < H && (uint32_t)b.w < W;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) box_sums_kernel(const T* __restrict__ sat, int64_t pitch, int64_t plane_stride,
+                                                       uint32_t W, uint32_t H, uint64_t count, uint32_t nbox,
+                                                       const int4* __restrict__ boxes, T* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t p = i / nbox;
+  const int4 b = boxes[i];
+  if (!box_ok(b, W, H)) {
+    out[i] = __builtin_nan("");
+    return;
+  }
+  const T* s = sat + p * plane_stride;
+  const T s11 = s[(int64_t)b.z * pitch + b.w];
+  const T s01 = b.x > 0 ? s[(int64_t)(b.x - 1) * pitch + b.w] : T(0);
+  const T s10 = b.y > 0 ? s[(int64_t)b.z * pitch + (b.y - 1)] : T(0);
+  const T s00 = (b.x > 0 && b.y > 0) ? s[(int64_t)(b.x - 1) * pitch + (b.y - 1)] : T(0);
+  out[i] = ((s11 - s01) - s10) + s00;
+}
+
+template <typename T> struct GradCfg;
+template <> struct GradCfg<float> { static constexpr int NW = 4, RPT = 8; };
+template <> struct GradCfg<double> { static constexpr int NW = 4, RPT = 4; };
+
+// grad_x = reverse SAT of the boxes' corner impulses (+g at (y1,x1) and
+// (y0-1,x0-1), -g at (y0-1,x1) and (y1,x0-1)). Each tile rebuilds its slice of
+// the impulse image in LDS from the plane's box list instead of reading a
+// dense impulse image, then runs the reverse-SAT tile operator.
+template <typename T>
+__global__ void __launch_bounds__(64 * GradCfg<T>::NW)
+    box_grad_kernel(const int4* __restrict__ boxes, const T* __restrict__ gout, uint32_t nbox, ImgArgs g,
+                    uint32_t tpp, int64_t out_plane, WsHeader* hdr, uint64_t* carries, uint64_t words_used) {
+  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  constexpr int TH = Op::TH, TW = Op::TW, NT = Op::NT, RPT = GradCfg<T>::RPT;
+  union Smem {
+    typename Op::Smem c;
+    T imp[TH * TW];
+  };
+  __shared__ Smem sm;
+  __shared__ uint32_t s_ticket, s_ep, s_flag;
+  if (threadIdx.x == 0) {
+    s_ticket = take_ticket(hdr);
+    s_ep = read_epoch(hdr);
+  }
+  __syncthreads();
+  const uint32_t t = s_ticket;
+  const uint32_t p = t / tpp, lt = t - p * tpp;
+  ImgArgs gp = g;
+  gp.out = (T*)g.out + (int64_t)p * out_plane;
+  gp.vcarry = g.vcarry + (int64_t)p * tpp * Op::VWORDS;
+  gp.hcarry = g.hcarry + (int64_t)p * tpp * Op::HWORDS;
+  const typename Op::Coords k = Op::coords(gp, lt);
+
+  for (int i = threadIdx.x; i < TH * TW; i += NT) sm.imp[i] = T(0);
+  __syncthreads();
+  const int4* pb = boxes + (uint64_t)p * nbox;
+  const T* pg = gout + (uint64_t)p * nbox;
+  for (uint32_t b = threadIdx.x; b < nbox; b += NT) {
+    const int4 bx = pb[b];
+    if (!box_ok(bx, g.W, g.H)) continue;
+    const T gv = pg[b];
+    const int64_t ys[2] = {bx.z, (int64_t)bx.x - 1};
+    const int64_t xs[2] = {bx.w, (int64_t)bx.y - 1};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int64_t ly = ys[a] - k.py0, lx = xs[c] - k.px0;
+        if (ys[a] >= 0 && xs[c] >= 0 && ly >= 0 && ly < TH && lx >= 0 && lx < TW)
+          atomicAdd(&sm.imp[ly * TW + lx], (a ^ c) ? -gv : gv);
+      }
+  }
+  __syncthreads();
+  T v[RPT][4];
+  const int64_t lcol = k.pcol - k.px0;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int64_t ly = Op::prow_of(k, r) - k.py0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[r][c] = sm.imp[ly * TW + lcol + 3 - c];
+  }
+  __syncthreads();
+  Op::finish(gp, lt, s_ep << 2, sm.c, k, v);
+  finish_launch(hdr, carries, words_used, &s_flag);
+}
+
+}  // namespace satdev
+
+namespace sathost {
+
+using namespace satdev;
+
+sat_status launch_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width,
+                           uint32_t height, uint32_t planes, const int32_t* boxes, uint32_t nbox, void* out,
+                           hipStream_t stream) {
+  const uint64_t count = (uint64_t)planes * nbox;
+  if (count == 0) return SAT_OK;
+  const uint64_t blocks = (count + 255) / 256;
+  if (blocks > 0x7fffffffull) return SAT_ERR_INVALID;
+  if (dtype == SAT_F32)
+    hipLaunchKernelGGL(box_sums_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream, (const float*)sat, pitch,
+                       plane_stride, width, height, count, nbox, (const int4*)boxes, (float*)out);
+  else if (dtype == SAT_F64)
+    hipLaunchKernelGGL(box_sums_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream, (const double*)sat,
+                       pitch, plane_stride, width, height, count, nbox, (const int4*)boxes, (double*)out);
+  else
+    return SAT_ERR_DTYPE;
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
+
+template <typename T>
+static uint64_t grad_words(uint32_t width, uint32_t height, uint32_t planes) {
+  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  const uint64_t nTx = (width + Op::TW - 1) / Op::TW, nTy = (height + Op::TH - 1) / Op::TH;
+  return nTx * nTy * planes * (uint64_t)(Op::VWORDS + Op::HWORDS);
+}
+
+size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes) {
+  const uint64_t w = dtype == SAT_F64 ? grad_words<double>(width, height, planes) : grad_words<float>(width, height, planes);
+  return (size_t)(WS_HEADER_BYTES + w * 8);
+}
+
+template <typename T>
+static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
+                              int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes, void* ws,
+                              size_t ws_bytes, hipStream_t stream) {
+  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  const uint64_t nTx = (width + Op::TW - 1) / Op::TW, nTy = (height + Op::TH - 1) / Op::TH;
+  const uint64_t tpp = nTx * nTy, ntiles = tpp * planes;
+  if (ntiles == 0) return SAT_OK;
+  if (ntiles > 0x7fffffffull) return SAT_ERR_INVALID;
+  const uint64_t vw = ntiles * Op::VWORDS, hw = ntiles * Op::HWORDS;
+  if (ws_bytes < WS_HEADER_BYTES + (vw + hw) * 8) return SAT_ERR_WORKSPACE;
+  uint64_t* carries = (uint64_t*)((char*)ws + WS_HEADER_BYTES);
+  ImgArgs g;
+  g.in = nullptr;
+  g.out = grad_x;
+  g.ldi = 0;
+  g.ldo = pitch;
+  g.H = height;
+  g.W = width;
+  g.nTx = (uint32_t)nTx;
+  g.nTy = (uint32_t)nTy;
+  g.vcarry = carries;
+  g.hcarry = carries + vw;
+  g.vin = 0;
+  g.vout = aligned_for(grad_x, pitch, plane_stride, sizeof(T)) ? 1u : 0u;
+  hipLaunchKernelGGL(box_grad_kernel<T>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, (const int4*)boxes,
+                     (const T*)grad_out, nbox, g, (uint32_t)tpp, plane_stride, (WsHeader*)ws, carries, vw + hw);
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
+
+sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x,
+                           int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
+                           void* ws, size_t ws_bytes, hipStream_t stream) {
+  if (dtype == SAT_F32)
+    return grad_launch<float>(boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, ws, ws_bytes,
+                              stream);
+  if (dtype == SAT_F64)
+    return grad_launch<double>(boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, ws, ws_bytes,
+                               stream);
+  return SAT_ERR_DTYPE;
+}
+
+}  // namespace sathost

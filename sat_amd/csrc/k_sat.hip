@@ -1,0 +1,62 @@
+// One-pass SAT kernels (forward and reverse) and the family geometry table.
+#include "sat_launch.hpp"
+
+namespace sathost {
+
+using namespace satdev;
+
+template <typename S, typename D, int SC>
+struct FamSat {
+  using Op = TileOp<S, D, MODE_SAT, false, false, SC, kNW, kRPT<D>>;
+  static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
+};
+
+template <typename S, typename D, int SC>
+struct FamSatRev {
+  using Op = TileOp<S, D, MODE_SAT, true, false, SC, kNW, kRPT<D>>;
+  static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
+};
+
+sat_status launch_fam_sat(const Launch& L, void* ws, size_t b, hipStream_t st) {
+  return dispatch_pairs<FamSat>(L, ws, b, st);
+}
+
+// The adjoint is defined for same-type pairs.
+sat_status launch_fam_sat_rev(const Launch& L, void* ws, size_t b, hipStream_t st) {
+  if (L.tin != L.tout) return SAT_ERR_DTYPE;
+  switch (L.tin) {
+    case SAT_F32: return ScanPick<true>::go<FamSatRev, float, float>(L, ws, b, st);
+    case SAT_F64: return ScanPick<true>::go<FamSatRev, double, double>(L, ws, b, st);
+    case SAT_U32: return ScanPick<false>::go<FamSatRev, uint32_t, uint32_t>(L, ws, b, st);
+    case SAT_I32: return ScanPick<false>::go<FamSatRev, int32_t, int32_t>(L, ws, b, st);
+    default: return SAT_ERR_DTYPE;
+  }
+}
+
+Geometry geometry(int family, size_t out_size) {
+  const bool b8 = out_size == 8;
+  switch (family) {
+    case FAM_SAT:
+    case FAM_SAT_REV:
+      return b8 ? geom_of<TileOp<double, double, MODE_SAT, false, false, 0, kNW, kRPT<double>>>()
+                : geom_of<TileOp<float, float, MODE_SAT, false, false, 0, kNW, kRPT<float>>>();
+    case FAM_ROWS:
+      return b8 ? geom_of<TileOp<double, double, MODE_ROW, false, false, 0, kNW, kRPT<double>>>()
+                : geom_of<TileOp<float, float, MODE_ROW, false, false, 0, kNW, kRPT<float>>>();
+    case FAM_ROWS_T:
+      return b8 ? geom_of<TileOp<double, double, MODE_ROW, false, true, 0, kNW, kRPT<double>>>()
+                : geom_of<TileOp<float, float, MODE_ROW, false, true, 0, kNW, kRPT<float>>>();
+    case FAM_COLS:
+      return b8 ? geom_of<TileOp<double, double, MODE_COL, false, false, 0, kNW, kRPT<double>>>()
+                : geom_of<TileOp<float, float, MODE_COL, false, false, 0, kNW, kRPT<float>>>();
+    case FAM_SERIAL_T:
+      return b8 ? geom_of<SerialTileOp<double, double, kNW, kSerialCW<double>>>()
+                : geom_of<SerialTileOp<float, float, kNW, kSerialCW<float>>>();
+    default: {
+      Geometry g{0, 0, 0, 0, 0};
+      return g;
+    }
+  }
+}
+
+}  // namespace sathost

@@ -1,0 +1,540 @@
+// C ABI of the MI355X summed-area-table library: argument validation,
+// workspaces, two-pass compositions and the batch planner. Kernels live in
+// k_*.hip; nothing here falls back to host arithmetic.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "sat_host.hpp"
+
+namespace sathost {
+
+sat_status launch_fam_sat(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_rows(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_rows_t(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_cols(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_serial_t(const Launch& L, void* ws, size_t b, hipStream_t st);
+
+size_t dtype_size(int dt) {
+  switch (dt) {
+    case SAT_U8:
+    case SAT_I8: return 1;
+    case SAT_U16:
+    case SAT_I16: return 2;
+    case SAT_U32:
+    case SAT_I32:
+    case SAT_F32: return 4;
+    case SAT_F64: return 8;
+    default: return 0;
+  }
+}
+
+static uint64_t tiles_of(const Geometry& g, uint32_t width, uint32_t height, uint32_t planes) {
+  if (g.th == 0 || g.tw == 0) return 0;
+  return (uint64_t)((width + g.tw - 1) / g.tw) * ((height + g.th - 1) / g.th) * planes;
+}
+
+size_t carry_bytes(const Launch& L) {
+  const Geometry g = geometry(L.family, dtype_size(L.tout));
+  const uint64_t n = tiles_of(g, L.width, L.height, L.planes);
+  return (size_t)(satdev::WS_HEADER_BYTES + n * (uint64_t)(g.vwords + g.hwords) * 8);
+}
+
+sat_status launch_family(const Launch& L, void* ws, size_t ws_bytes, hipStream_t st) {
+  switch (L.family) {
+    case FAM_SAT: return launch_fam_sat(L, ws, ws_bytes, st);
+    case FAM_SAT_REV: return launch_fam_sat_rev(L, ws, ws_bytes, st);
+    case FAM_ROWS: return launch_fam_rows(L, ws, ws_bytes, st);
+    case FAM_ROWS_T: return launch_fam_rows_t(L, ws, ws_bytes, st);
+    case FAM_COLS: return launch_fam_cols(L, ws, ws_bytes, st);
+    case FAM_SERIAL_T: return launch_fam_serial_t(L, ws, ws_bytes, st);
+    default: return SAT_ERR_INVALID;
+  }
+}
+
+int device_cu_count() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) return 1;
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Library-managed workspaces, one per (device, stream); grown on demand.
+// ---------------------------------------------------------------------------
+struct ManagedWs {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+static std::mutex g_ws_mu;
+static std::map<std::pair<int, uintptr_t>, ManagedWs> g_ws;
+
+static sat_status managed_ws(hipStream_t st, size_t need, void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return SAT_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  ManagedWs& m = g_ws[{dev, (uintptr_t)st}];
+  if (m.bytes < need) {
+    if (m.ptr) {
+      if (hipStreamSynchronize(st) != hipSuccess) return SAT_ERR_HIP;
+      (void)hipFree(m.ptr);
+      m.ptr = nullptr;
+      m.bytes = 0;
+    }
+    size_t nb = std::max(need, (size_t)1 << 20);
+    nb = (nb + 0xffff) & ~(size_t)0xffff;
+    void* p = nullptr;
+    if (hipMalloc(&p, nb) != hipSuccess) return SAT_ERR_HIP;
+    if (hipMemsetAsync(p, 0, nb, st) != hipSuccess) {
+      (void)hipFree(p);
+      return SAT_ERR_HIP;
+    }
+    m.ptr = p;
+    m.bytes = nb;
+  }
+  *out = m.ptr;
+  return SAT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Validation helpers.
+// ---------------------------------------------------------------------------
+static bool pair_ok(int tin, int tout) { return pair_id(tin, tout) >= 0; }
+static bool scan_ok(int scan) { return scan == SAT_SCAN_KOGGE_STONE || scan == SAT_SCAN_LF; }
+
+static sat_status check_2d(const void* in, const void* out, int64_t ldi, int64_t ldo, uint32_t width,
+                           uint32_t height, bool transposed_out) {
+  if (width == 0 || height == 0) return SAT_OK;
+  if (!in || !out) return SAT_ERR_NULL;
+  if (ldi < (int64_t)width) return SAT_ERR_INVALID;
+  if (ldo < (int64_t)(transposed_out ? height : width)) return SAT_ERR_INVALID;
+  return SAT_OK;
+}
+
+static sat_status run_with_ws(const Launch& L, void* ws, size_t ws_bytes, hipStream_t st) {
+  const size_t need = carry_bytes(L);
+  if (!ws) {
+    const sat_status s = managed_ws(st, need, &ws);
+    if (s != SAT_OK) return s;
+    ws_bytes = need;
+  }
+  return launch_family(L, ws, ws_bytes, st);
+}
+
+static Launch make_launch(int family, int scan, int tin, int tout, const void* in, int64_t ldi, void* out, int64_t ldo,
+                          uint32_t width, uint32_t height) {
+  Launch L;
+  L.family = family;
+  L.scan = scan;
+  L.tin = tin;
+  L.tout = tout;
+  L.in = in;
+  L.out = out;
+  L.ldi = ldi;
+  L.ldo = ldo;
+  L.in_plane = 0;
+  L.out_plane = 0;
+  L.width = width;
+  L.height = height;
+  L.planes = 1;
+  return L;
+}
+
+static sat_status pass_call(int family, int scan, int tin, int tout, const void* in, int64_t ldi, void* out,
+                            int64_t ldo, uint32_t width, uint32_t height, void* stream) {
+  if (!scan_ok(scan)) return SAT_ERR_INVALID;
+  if (!pair_ok(tin, tout)) return SAT_ERR_DTYPE;
+  const bool tr = family == FAM_ROWS_T || family == FAM_SERIAL_T;
+  const sat_status c = check_2d(in, out, ldi, ldo, width, height, tr);
+  if (c != SAT_OK || width == 0 || height == 0) return c;
+  return run_with_ws(make_launch(family, scan, tin, tout, in, ldi, out, ldo, width, height), nullptr, 0,
+                     (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Full SAT: workspace layout = carries (max over the passes) | temporary.
+// ---------------------------------------------------------------------------
+struct SatPlan {
+  Launch p1, p2;   // p2.family < 0 for the one-pass algorithm
+  size_t carry;    // bytes of header + carries
+  size_t tmp_off;  // byte offset of the temporary
+  size_t total;    // workspace bytes
+  int64_t tmp_pitch, tmp_plane;
+};
+
+static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+static sat_status plan_integral(int alg, int scan, int tin, int tout, const void* in, int64_t ldi, int64_t in_plane,
+                                void* out, int64_t ldo, int64_t out_plane, uint32_t width, uint32_t height,
+                                uint32_t planes, SatPlan* P) {
+  if (!scan_ok(scan)) return SAT_ERR_INVALID;
+  if (!pair_ok(tin, tout)) return SAT_ERR_DTYPE;
+  const size_t osz = dtype_size(tout);
+  Launch a = make_launch(FAM_SAT, scan, tin, tout, in, ldi, out, ldo, width, height);
+  a.in_plane = in_plane;
+  a.out_plane = out_plane;
+  a.planes = planes;
+  Launch b = a;
+  b.family = -1;
+  P->tmp_pitch = 0;
+  P->tmp_plane = 0;
+  switch (alg) {
+    case SAT_ALG_SINGLE_PASS:
+      break;
+    case SAT_ALG_SCANROW_BRLT:
+    case SAT_ALG_BRLT_SCANROW: {
+      // pass 1: in (H x W) -> tmp (W x H); pass 2: tmp -> out (H x W)
+      const int fam = alg == SAT_ALG_SCANROW_BRLT ? FAM_ROWS_T : FAM_SERIAL_T;
+      P->tmp_pitch = round_up(height, 64);
+      P->tmp_plane = P->tmp_pitch * width;
+      a.family = fam;
+      a.out = nullptr;
+      a.ldo = P->tmp_pitch;
+      a.out_plane = P->tmp_plane;
+      b = make_launch(fam, scan, tout, tout, nullptr, P->tmp_pitch, out, ldo, height, width);
+      b.in_plane = P->tmp_plane;
+      b.out_plane = out_plane;
+      b.planes = planes;
+      break;
+    }
+    case SAT_ALG_SCANROW_COLUMN: {
+      P->tmp_pitch = round_up(width, 64);
+      P->tmp_plane = P->tmp_pitch * height;
+      a.family = FAM_ROWS;
+      a.out = nullptr;
+      a.ldo = P->tmp_pitch;
+      a.out_plane = P->tmp_plane;
+      b = make_launch(FAM_COLS, scan, tout, tout, nullptr, P->tmp_pitch, out, ldo, width, height);
+      b.in_plane = P->tmp_plane;
+      b.out_plane = out_plane;
+      b.planes = planes;
+      break;
+    }
+    default:
+      return SAT_ERR_INVALID;
+  }
+  P->p1 = a;
+  P->p2 = b;
+  size_t carry = carry_bytes(a);
+  if (b.family >= 0) carry = std::max(carry, carry_bytes(b));
+  P->carry = carry;
+  P->tmp_off = (size_t)round_up((int64_t)carry, 256);
+  P->total = P->tmp_off + (b.family >= 0 ? (size_t)P->tmp_plane * planes * osz : 0);
+  return SAT_OK;
+}
+
+static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (!ws) {
+    const sat_status s = managed_ws(st, P.total, &ws);
+    if (s != SAT_OK) return s;
+    ws_bytes = P.total;
+  }
+  if (ws_bytes < P.total) return SAT_ERR_WORKSPACE;
+  if (P.p2.family < 0) return launch_family(P.p1, ws, ws_bytes, st);
+  void* tmp = (char*)ws + P.tmp_off;
+  P.p1.out = tmp;
+  P.p2.in = tmp;
+  sat_status s = launch_family(P.p1, ws, P.carry, st);
+  if (s != SAT_OK) return s;
+  return launch_family(P.p2, ws, P.carry, st);
+}
+
+// ---------------------------------------------------------------------------
+// Batch plans. Layout in the workspace:
+//   [0, 256)          launch-state header
+//   [256, items)      carry words
+//   items             BatchItem[count]
+//   tile_img          uint32_t[total_tiles]
+// ---------------------------------------------------------------------------
+struct BatchPlanInfo {
+  int out_size = 0;
+  uint32_t total_tiles = 0;
+  uint64_t carry_words = 0;
+  size_t items_off = 0, tile_img_off = 0, total = 0;
+};
+static std::mutex g_batch_mu;
+static std::map<uintptr_t, BatchPlanInfo> g_batch;
+
+static sat_status batch_layout(const sat_image* im, int count, BatchPlanInfo* info, std::vector<BatchItem>* items,
+                               std::vector<uint32_t>* tile_img) {
+  if (count < 0) return SAT_ERR_INVALID;
+  if (count > 0 && !im) return SAT_ERR_NULL;
+  int osz = 0;
+  for (int i = 0; i < count; ++i) {
+    if (!pair_ok(im[i].tin, im[i].tout)) return SAT_ERR_DTYPE;
+    const int s = (int)dtype_size(im[i].tout);
+    if (osz == 0) osz = s;
+    if (s != osz) return SAT_ERR_DTYPE;
+    const sat_status c =
+        check_2d(im[i].in, im[i].out, im[i].in_pitch, im[i].out_pitch, im[i].width, im[i].height, false);
+    if (c != SAT_OK) return c;
+  }
+  if (osz == 0) osz = 4;
+  const Geometry g = batch_geometry(osz);
+  uint64_t tiles = 0;
+  for (int i = 0; i < count; ++i) tiles += tiles_of(g, im[i].width, im[i].height, 1);
+  if (tiles > 0x7fffffffull) return SAT_ERR_INVALID;
+  info->out_size = osz;
+  info->total_tiles = (uint32_t)tiles;
+  info->carry_words = tiles * (uint64_t)(g.vwords + g.hwords);
+  info->items_off = (size_t)round_up(satdev::WS_HEADER_BYTES + (int64_t)info->carry_words * 8, 256);
+  info->tile_img_off = (size_t)round_up((int64_t)(info->items_off + sizeof(BatchItem) * (size_t)count), 256);
+  info->total = info->tile_img_off + sizeof(uint32_t) * (size_t)tiles;
+  if (!items) return SAT_OK;
+  items->resize(count);
+  tile_img->resize(tiles);
+  uint32_t base = 0;
+  for (int i = 0; i < count; ++i) {
+    BatchItem& it = (*items)[i];
+    const uint32_t nTx = (im[i].width + g.tw - 1) / g.tw, nTy = (im[i].height + g.th - 1) / g.th;
+    it.g.in = im[i].in;
+    it.g.out = im[i].out;
+    it.g.ldi = im[i].in_pitch;
+    it.g.ldo = im[i].out_pitch;
+    it.g.H = im[i].height;
+    it.g.W = im[i].width;
+    it.g.nTx = nTx;
+    it.g.nTy = nTy;
+    it.g.vcarry = nullptr;  // rebased on the device address in sat_batch_prepare
+    it.g.hcarry = nullptr;
+    it.g.vin = 0;
+    it.g.vout = 0;
+    {
+      const size_t si = dtype_size(im[i].tin), so = dtype_size(im[i].tout);
+      const size_t ni = si * 4 >= 16 ? 16 : si * 4, no = so * 4 >= 16 ? 16 : so * 4;
+      it.g.vin = ((uintptr_t)im[i].in % ni == 0 && ((uint64_t)im[i].in_pitch * si) % ni == 0) ? 1u : 0u;
+      it.g.vout = ((uintptr_t)im[i].out % no == 0 && ((uint64_t)im[i].out_pitch * so) % no == 0) ? 1u : 0u;
+    }
+    it.kind = (uint32_t)pair_id(im[i].tin, im[i].tout);
+    it.tile_base = base;
+    const uint32_t n = nTx * nTy;
+    for (uint32_t k = 0; k < n; ++k) (*tile_img)[base + k] = (uint32_t)i;
+    base += n;
+  }
+  return SAT_OK;
+}
+
+}  // namespace sathost
+
+using namespace sathost;
+
+extern "C" {
+
+int sat_abi_version(void) { return SAT_ABI_VERSION; }
+
+size_t sat_dtype_size(int dtype) { return dtype_size(dtype); }
+
+int sat_dtype_pair_supported(int tin, int tout) { return pair_id(tin, tout) >= 0 ? 1 : 0; }
+
+const char* sat_status_string(int status) {
+  switch (status) {
+    case SAT_OK: return "ok";
+    case SAT_ERR_NULL: return "null pointer";
+    case SAT_ERR_INVALID: return "invalid argument";
+    case SAT_ERR_DTYPE: return "unsupported dtype pair";
+    case SAT_ERR_WORKSPACE: return "workspace too small";
+    case SAT_ERR_HIP: return "HIP runtime error";
+    case SAT_ERR_UNSUPPORTED: return "not supported by this build";
+    default: return "unknown status";
+  }
+}
+
+int sat_block_scan(int scan, int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
+                   uint32_t height, uint32_t heightStride, void* stream) {
+  return pass_call(FAM_ROWS_T, scan, tin, tout, in, widthStride, out, heightStride, width, height, stream);
+}
+
+int sat_seriel_scan(int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
+                    uint32_t height, uint32_t heightStride, void* stream) {
+  return pass_call(FAM_SERIAL_T, SAT_SCAN_KOGGE_STONE, tin, tout, in, widthStride, out, heightStride, width, height,
+                   stream);
+}
+
+int sat_scan_x(int scan, int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
+               uint32_t height, void* stream) {
+  return pass_call(FAM_ROWS, scan, tin, tout, in, widthStride, out, widthStride, width, height, stream);
+}
+
+int sat_scan_y(int scan, int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
+               uint32_t height, void* stream) {
+  return pass_call(FAM_COLS, scan, tin, tout, in, widthStride, out, widthStride, width, height, stream);
+}
+
+int sat_scan_rows(int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                  uint32_t width, uint32_t height, void* stream) {
+  return pass_call(FAM_ROWS, scan, tin, tout, in, in_pitch, out, out_pitch, width, height, stream);
+}
+
+int sat_scan_cols(int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                  uint32_t width, uint32_t height, void* stream) {
+  return pass_call(FAM_COLS, scan, tin, tout, in, in_pitch, out, out_pitch, width, height, stream);
+}
+
+size_t sat_integral_planes_workspace_bytes(int alg, int tin, int tout, uint32_t width, uint32_t height,
+                                           uint32_t planes) {
+  SatPlan P;
+  int dummy = 0;
+  if (plan_integral(alg, SAT_SCAN_KOGGE_STONE, tin, tout, &dummy, width, (int64_t)width * height, &dummy, width,
+                    (int64_t)width * height, width, height, planes, &P) != SAT_OK)
+    return 0;
+  return P.total;
+}
+
+size_t sat_integral_workspace_bytes(int alg, int tin, int tout, uint32_t width, uint32_t height) {
+  return sat_integral_planes_workspace_bytes(alg, tin, tout, width, height, 1);
+}
+
+int sat_integral_planes(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch,
+                        int64_t in_plane_stride, void* out, int64_t out_pitch, int64_t out_plane_stride,
+                        uint32_t width, uint32_t height, uint32_t planes, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  if (!scan_ok(scan) || alg < 0 || alg > SAT_ALG_SINGLE_PASS) return SAT_ERR_INVALID;
+  if (!pair_ok(tin, tout)) return SAT_ERR_DTYPE;
+  if (width == 0 || height == 0 || planes == 0) return SAT_OK;
+  sat_status c = check_2d(in, out, in_pitch, out_pitch, width, height, false);
+  if (c != SAT_OK) return c;
+  if (planes > 1 && (in_plane_stride < in_pitch * (int64_t)height || out_plane_stride < out_pitch * (int64_t)height))
+    return SAT_ERR_INVALID;
+  SatPlan P;
+  c = plan_integral(alg, scan, tin, tout, in, in_pitch, in_plane_stride, out, out_pitch, out_plane_stride, width,
+                    height, planes, &P);
+  if (c != SAT_OK) return c;
+  return run_integral(P, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int sat_integral(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                 uint32_t width, uint32_t height, void* workspace, size_t workspace_bytes, void* stream) {
+  return sat_integral_planes(alg, scan, tin, tout, in, in_pitch, 0, out, out_pitch, 0, width, height, 1, workspace,
+                             workspace_bytes, stream);
+}
+
+size_t sat_integral_reverse_planes_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes) {
+  Launch L = make_launch(FAM_SAT_REV, 0, dtype, dtype, nullptr, width, nullptr, width, width, height);
+  L.planes = planes;
+  return carry_bytes(L);
+}
+
+size_t sat_integral_reverse_workspace_bytes(int dtype, uint32_t width, uint32_t height) {
+  return sat_integral_reverse_planes_workspace_bytes(dtype, width, height, 1);
+}
+
+int sat_integral_reverse_planes(int scan, int dtype, const void* in, int64_t in_pitch, int64_t in_plane_stride,
+                                void* out, int64_t out_pitch, int64_t out_plane_stride, uint32_t width,
+                                uint32_t height, uint32_t planes, void* workspace, size_t workspace_bytes,
+                                void* stream) {
+  if (!scan_ok(scan)) return SAT_ERR_INVALID;
+  if (!(dtype == SAT_F32 || dtype == SAT_F64 || dtype == SAT_U32 || dtype == SAT_I32)) return SAT_ERR_DTYPE;
+  if (width == 0 || height == 0 || planes == 0) return SAT_OK;
+  const sat_status c = check_2d(in, out, in_pitch, out_pitch, width, height, false);
+  if (c != SAT_OK) return c;
+  if (planes > 1 && (in_plane_stride < in_pitch * (int64_t)height || out_plane_stride < out_pitch * (int64_t)height))
+    return SAT_ERR_INVALID;
+  Launch L = make_launch(FAM_SAT_REV, scan, dtype, dtype, in, in_pitch, out, out_pitch, width, height);
+  L.in_plane = in_plane_stride;
+  L.out_plane = out_plane_stride;
+  L.planes = planes;
+  return run_with_ws(L, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int sat_integral_reverse(int scan, int dtype, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                         uint32_t width, uint32_t height, void* workspace, size_t workspace_bytes, void* stream) {
+  return sat_integral_reverse_planes(scan, dtype, in, in_pitch, 0, out, out_pitch, 0, width, height, 1, workspace,
+                                     workspace_bytes, stream);
+}
+
+size_t sat_batch_workspace_bytes(const sat_image* images, int count) {
+  BatchPlanInfo info;
+  if (batch_layout(images, count, &info, nullptr, nullptr) != SAT_OK) return 0;
+  return info.total;
+}
+
+int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!workspace) return SAT_ERR_NULL;
+  BatchPlanInfo info;
+  std::vector<BatchItem> items;
+  std::vector<uint32_t> tile_img;
+  sat_status s = batch_layout(images, count, &info, &items, &tile_img);
+  if (s != SAT_OK) return s;
+  if (workspace_bytes < info.total) return SAT_ERR_WORKSPACE;
+  const Geometry g = batch_geometry(info.out_size);
+  uint64_t* carries = (uint64_t*)((char*)workspace + satdev::WS_HEADER_BYTES);
+  for (auto& it : items) {
+    it.g.vcarry = carries + (uint64_t)it.tile_base * g.vwords;
+    it.g.hcarry = carries + (uint64_t)info.total_tiles * g.vwords + (uint64_t)it.tile_base * g.hwords;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (hipMemsetAsync(workspace, 0, info.items_off, st) != hipSuccess) return SAT_ERR_HIP;
+  if (!items.empty() &&
+      hipMemcpyAsync((char*)workspace + info.items_off, items.data(), sizeof(BatchItem) * items.size(),
+                     hipMemcpyHostToDevice, st) != hipSuccess)
+    return SAT_ERR_HIP;
+  if (!tile_img.empty() &&
+      hipMemcpyAsync((char*)workspace + info.tile_img_off, tile_img.data(), sizeof(uint32_t) * tile_img.size(),
+                     hipMemcpyHostToDevice, st) != hipSuccess)
+    return SAT_ERR_HIP;
+  if (hipStreamSynchronize(st) != hipSuccess) return SAT_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  g_batch[(uintptr_t)workspace] = info;
+  return SAT_OK;
+}
+
+int sat_batch_run(void* workspace, int ctas_per_cu, void* stream) {
+  if (!workspace) return SAT_ERR_NULL;
+  BatchPlanInfo info;
+  {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    auto f = g_batch.find((uintptr_t)workspace);
+    if (f == g_batch.end()) return SAT_ERR_INVALID;
+    info = f->second;
+  }
+  if (info.total_tiles == 0) return SAT_OK;
+  if (ctas_per_cu < 0) return SAT_ERR_INVALID;
+  const int bpc = ctas_per_cu > 0 ? ctas_per_cu : batch_blocks_per_cu(info.out_size);
+  int64_t grid = (int64_t)bpc * device_cu_count();
+  if (grid > (int64_t)info.total_tiles) grid = info.total_tiles;
+  uint64_t* carries = (uint64_t*)((char*)workspace + satdev::WS_HEADER_BYTES);
+  return launch_batch(info.out_size, (const BatchItem*)((char*)workspace + info.items_off),
+                      (const uint32_t*)((char*)workspace + info.tile_img_off), info.total_tiles, workspace, carries,
+                      info.carry_words, (int)grid, (hipStream_t)stream);
+}
+
+int sat_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height,
+                 uint32_t planes, const int32_t* boxes, uint32_t nbox, void* out, void* stream) {
+  if (dtype != SAT_F32 && dtype != SAT_F64) return SAT_ERR_DTYPE;
+  if ((uint64_t)planes * nbox == 0) return SAT_OK;
+  if (!sat || !boxes || !out) return SAT_ERR_NULL;
+  if (width == 0 || height == 0 || pitch < (int64_t)width) return SAT_ERR_INVALID;
+  if (planes > 1 && plane_stride < pitch * (int64_t)height) return SAT_ERR_INVALID;
+  return launch_box_sums(dtype, sat, pitch, plane_stride, width, height, planes, boxes, nbox, out,
+                         (hipStream_t)stream);
+}
+
+size_t sat_box_grad_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes, uint32_t nbox) {
+  (void)nbox;
+  if (dtype != SAT_F32 && dtype != SAT_F64) return 0;
+  return box_grad_carry_bytes(dtype, width, height, planes);
+}
+
+int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
+                 int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes, void* workspace,
+                 size_t workspace_bytes, void* stream) {
+  if (dtype != SAT_F32 && dtype != SAT_F64) return SAT_ERR_DTYPE;
+  if (width == 0 || height == 0 || planes == 0) return SAT_OK;
+  if (!grad_x || (nbox > 0 && (!boxes || !grad_out))) return SAT_ERR_NULL;
+  if (pitch < (int64_t)width) return SAT_ERR_INVALID;
+  if (planes > 1 && plane_stride < pitch * (int64_t)height) return SAT_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t need = box_grad_carry_bytes(dtype, width, height, planes);
+  if (!workspace) {
+    const sat_status s = managed_ws(st, need, &workspace);
+    if (s != SAT_OK) return s;
+    workspace_bytes = need;
+  }
+  return launch_box_grad(dtype, boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, workspace,
+                         workspace_bytes, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,369 @@
+// Device-side building blocks of the MI355X (gfx950) summed-area-table kernels.
+//
+// Everything here is wave64 code: cross-lane scans use DPP row shifts and the
+// GFX9 row broadcasts, tile carries travel through 64-bit "tagged" words whose
+// upper half holds (epoch << 2 | status) and whose lower half holds the payload.
+// A payload wider than 32 bits is split over two tagged words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace satdev {
+
+// ---------------------------------------------------------------------------
+// Arithmetic type. Integer outputs accumulate in unsigned 32-bit arithmetic so
+// that overflow wraps modulo 2^32 (the reference's `uint` accumulators,
+// integralImage.h:105-124, and the wrap the u32 / u8->u32 workloads rely on);
+// an int32 output is the same bit pattern reinterpreted.
+// ---------------------------------------------------------------------------
+template <typename T> struct arith { using type = T; };
+template <> struct arith<int32_t> { using type = uint32_t; };
+template <typename T> using arith_t = typename arith<T>::type;
+
+// TSrc -> TDst -> arithmetic type: the reference converts each pixel to TDst
+// before adding it (`sum += src[i]` with a TDst accumulator, integralImage.h:79).
+template <typename TSrc, typename TDst>
+__device__ __forceinline__ arith_t<TDst> cvt_in(TSrc x) {
+  return (arith_t<TDst>)(TDst)x;
+}
+template <typename TDst>
+__device__ __forceinline__ TDst cvt_out(arith_t<TDst> x) {
+  return (TDst)x;
+}
+
+template <typename A> __device__ __forceinline__ A zero() { return (A)0; }
+
+// ---------------------------------------------------------------------------
+// Cross-lane primitives (wave64).
+// ---------------------------------------------------------------------------
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+  // old = 0 and bound_ctrl: lanes with no source lane, or outside ROWMASK, read 0
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xf, true);
+}
+
+template <int CTRL, int ROWMASK, typename A>
+__device__ __forceinline__ A dpp(A x) {
+  if constexpr (sizeof(A) == 4) {
+    return __builtin_bit_cast(A, dpp32<CTRL, ROWMASK>(__builtin_bit_cast(uint32_t, x)));
+  } else {
+    static_assert(sizeof(A) == 8, "4- or 8-byte arithmetic types only");
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = dpp32<CTRL, ROWMASK>((uint32_t)b);
+    const uint32_t hi = dpp32<CTRL, ROWMASK>((uint32_t)(b >> 32));
+    return __builtin_bit_cast(A, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+// DPP controls (GFX9 encoding)
+constexpr int DPP_ROW_SHR1 = 0x111;
+constexpr int DPP_ROW_SHR2 = 0x112;
+constexpr int DPP_ROW_SHR4 = 0x114;
+constexpr int DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+// Inclusive scan over the 64 lanes of a wave: Hillis-Steele inside each
+// 16-lane DPP row, then the row totals are folded in with the two GFX9 row
+// broadcasts (lane 15 -> rows 1 and 3, lane 31 -> rows 2 and 3). This is the
+// Kogge-Stone order (the reference's KoggeStone_SCAN, ScanRow-BRLT.cu:58-67)
+// re-expressed for 64-wide waves.
+template <typename A>
+__device__ __forceinline__ A wave_scan_ks(A x) {
+  x += dpp<DPP_ROW_SHR1, 0xf>(x);
+  x += dpp<DPP_ROW_SHR2, 0xf>(x);
+  x += dpp<DPP_ROW_SHR4, 0xf>(x);
+  x += dpp<DPP_ROW_SHR8, 0xf>(x);
+  x += dpp<DPP_ROW_BCAST15, 0xa>(x);
+  x += dpp<DPP_ROW_BCAST31, 0xc>(x);
+  return x;
+}
+
+template <typename A>
+__device__ __forceinline__ uint32_t bits_lo(A x) {
+  if constexpr (sizeof(A) == 4) return __builtin_bit_cast(uint32_t, x);
+  else return (uint32_t)__builtin_bit_cast(uint64_t, x);
+}
+
+template <int PATTERN>
+__device__ __forceinline__ uint32_t swz32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, PATTERN);
+}
+template <int PATTERN, typename A>
+__device__ __forceinline__ A swz(A x) {
+  if constexpr (sizeof(A) == 4) {
+    return __builtin_bit_cast(A, swz32<PATTERN>(__builtin_bit_cast(uint32_t, x)));
+  } else {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = swz32<PATTERN>((uint32_t)b);
+    const uint32_t hi = swz32<PATTERN>((uint32_t)(b >> 32));
+    return __builtin_bit_cast(A, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+template <typename A>
+__device__ __forceinline__ A readlane(A x, int l) {
+  if constexpr (sizeof(A) == 4) {
+    return __builtin_bit_cast(A, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+  } else {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(A, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+// ds_swizzle bit-mask mode: source lane (within 32) = ((lane & and) | or) ^ xor
+constexpr int swz_mask(int and_mask, int or_mask, int xor_mask) {
+  return (and_mask & 0x1f) | ((or_mask & 0x1f) << 5) | ((xor_mask & 0x1f) << 10);
+}
+
+// Inclusive scan over 64 lanes in the Ladner-Fischer (Sklansky) order: at step
+// k every lane whose bit k is set adds the last lane of the preceding 2^k
+// block (the reference's LF_SCAN, ScanRow-BRLT.cu:68-76). Steps k = 0..4 are
+// ds_swizzle broadcasts inside 32-lane halves; step 5 broadcasts lane 31.
+template <typename A>
+__device__ __forceinline__ A wave_scan_lf(A x, int lane) {
+  A t;
+  t = swz<swz_mask(0x1e, 0x00, 0)>(x); if (lane & 1)  x += t;   // src = lane & ~1
+  t = swz<swz_mask(0x1c, 0x01, 0)>(x); if (lane & 2)  x += t;   // src = (lane & ~3) | 1
+  t = swz<swz_mask(0x18, 0x03, 0)>(x); if (lane & 4)  x += t;   // src = (lane & ~7) | 3
+  t = swz<swz_mask(0x10, 0x07, 0)>(x); if (lane & 8)  x += t;   // src = (lane & ~15) | 7
+  t = swz<swz_mask(0x00, 0x0f, 0)>(x); if (lane & 16) x += t;   // src = 15 (within the half)
+  t = readlane(x, 31);                  if (lane & 32) x += t;
+  return x;
+}
+
+template <int SCAN, typename A>
+__device__ __forceinline__ A wave_scan(A x, int lane) {
+  if constexpr (SCAN == 1) return wave_scan_lf(x, lane);
+  else return wave_scan_ks(x);
+}
+
+// exclusive value = inclusive value of the previous lane (lane 0 reads 0)
+template <typename A>
+__device__ __forceinline__ A wave_shr1(A x) {
+  return dpp<DPP_WAVE_SHR1, 0xf>(x);
+}
+
+// ---------------------------------------------------------------------------
+// Tagged carry words. tag = (epoch << 2) | status, status 1 = aggregate of the
+// tile alone, 2 = inclusive prefix through the tile. A word whose epoch is not
+// the current launch's reads as "not yet published".
+// ---------------------------------------------------------------------------
+constexpr uint32_t ST_AGG = 1;
+constexpr uint32_t ST_INC = 2;
+
+__device__ __forceinline__ uint64_t ld_word(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_word(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename A> struct Carry {
+  static constexpr int WORDS = sizeof(A) / 4;
+
+  __device__ static __forceinline__ void put(uint64_t* base, int64_t e, A v, uint32_t tag) {
+    const uint64_t t = (uint64_t)tag << 32;
+    if constexpr (WORDS == 1) {
+      st_word(base + e, t | __builtin_bit_cast(uint32_t, v));
+    } else {
+      const uint64_t b = __builtin_bit_cast(uint64_t, v);
+      st_word(base + 2 * e, t | (uint32_t)(b >> 32));
+      st_word(base + 2 * e + 1, t | (uint32_t)b);
+    }
+  }
+
+  // 0: not published in this launch yet, otherwise ST_AGG / ST_INC
+  __device__ static __forceinline__ uint32_t get(const uint64_t* base, int64_t e, uint32_t ep4, A& v) {
+    if constexpr (WORDS == 1) {
+      const uint64_t w = ld_word(base + e);
+      const uint32_t tag = (uint32_t)(w >> 32);
+      if ((tag & ~3u) != ep4) return 0;
+      v = __builtin_bit_cast(A, (uint32_t)w);
+      return tag & 3u;
+    } else {
+      const uint64_t h = ld_word(base + 2 * e);
+      const uint64_t l = ld_word(base + 2 * e + 1);
+      const uint32_t th = (uint32_t)(h >> 32), tl = (uint32_t)(l >> 32);
+      if (th != tl || (th & ~3u) != ep4) return 0;  // torn AGG/INC pair: retry
+      v = __builtin_bit_cast(A, ((uint64_t)(uint32_t)h << 32) | (uint32_t)l);
+      return th & 3u;
+    }
+  }
+};
+
+// Decoupled look-back over `n` predecessors, nearest first. `e` is the carry
+// entry of the nearest predecessor, `step` the entry distance to the next one.
+// Aggregates are summed until an inclusive prefix is found (or the chain ends).
+template <typename A>
+__device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t step, uint32_t n,
+                                       uint32_t ep4) {
+  A acc = zero<A>();
+  while (n > 0) {
+    A v;
+    const uint32_t st = Carry<A>::get(base, e, ep4, v);
+    if (st == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    acc += v;
+    if (st == ST_INC) break;
+    e -= step;
+    --n;
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// 4-wide global loads and stores of a row chunk [col, col+4) with zero fill
+// outside the image. `vec` selects one vector access when the chunk is whole
+// and the row base is aligned for it (decided on the host).
+// ---------------------------------------------------------------------------
+template <typename T> struct Vec4;
+template <> struct Vec4<uint8_t> { typedef uint8_t type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<int8_t> { typedef int8_t type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<uint32_t> { typedef uint32_t type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<int32_t> { typedef int32_t type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<double> { typedef double type __attribute__((ext_vector_type(4))); };
+
+template <typename TSrc, typename TDst>
+__device__ __forceinline__ void load4(const TSrc* __restrict__ row, int64_t col, int64_t W, bool row_ok,
+                                      bool vec, arith_t<TDst> (&x)[4]) {
+  using A = arith_t<TDst>;
+  if (row_ok && vec && col + 3 < W) {
+    const typename Vec4<TSrc>::type q = *(const typename Vec4<TSrc>::type*)(row + col);
+    x[0] = cvt_in<TSrc, TDst>(q.x);
+    x[1] = cvt_in<TSrc, TDst>(q.y);
+    x[2] = cvt_in<TSrc, TDst>(q.z);
+    x[3] = cvt_in<TSrc, TDst>(q.w);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      x[c] = (row_ok && col + c < W) ? cvt_in<TSrc, TDst>(row[col + c]) : zero<A>();
+  }
+}
+
+template <typename TDst>
+__device__ __forceinline__ void store4(TDst* __restrict__ row, int64_t col, int64_t W, bool row_ok, bool vec,
+                                       const arith_t<TDst> (&y)[4]) {
+  if (!row_ok) return;
+  if (vec && col + 3 < W) {
+    typename Vec4<TDst>::type q;
+    q.x = cvt_out<TDst>(y[0]);
+    q.y = cvt_out<TDst>(y[1]);
+    q.z = cvt_out<TDst>(y[2]);
+    q.w = cvt_out<TDst>(y[3]);
+    *(typename Vec4<TDst>::type*)(row + col) = q;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (col + c < W) row[col + c] = cvt_out<TDst>(y[c]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// XOR-swizzled LDS staging of a TW-wide tile for transposes. Element (i, j)
+// lives at i*TW + (j ^ (i & M)), M = 31 for 4-byte and 15 for 8-byte values,
+// so a wave reading one column (consecutive i) touches distinct banks, while a
+// lane's 4 consecutive elements stay inside one aligned 4-element chunk and are
+// written with a single vector store after an in-register permutation.
+// ---------------------------------------------------------------------------
+template <typename A> constexpr int swz_m() { return sizeof(A) == 4 ? 31 : 15; }
+
+template <typename A, int TW>
+__device__ __forceinline__ void stage_put4(A* st, int i, int j0, const A (&x)[4]) {
+  static_assert(TW % 32 == 0, "staging width");
+  const int s = i & swz_m<A>();
+  const int p = s & 3;
+  typename Vec4<A>::type q;
+  // q[d] = x[d ^ p]
+  q.x = p == 0 ? x[0] : p == 1 ? x[1] : p == 2 ? x[2] : x[3];
+  q.y = p == 0 ? x[1] : p == 1 ? x[0] : p == 2 ? x[3] : x[2];
+  q.z = p == 0 ? x[2] : p == 1 ? x[3] : p == 2 ? x[0] : x[1];
+  q.w = p == 0 ? x[3] : p == 1 ? x[2] : p == 2 ? x[1] : x[0];
+  *(typename Vec4<A>::type*)(st + i * TW + (j0 ^ (s & ~3))) = q;
+}
+
+template <typename A, int TW>
+__device__ __forceinline__ A stage_get(const A* st, int i, int j) {
+  return st[i * TW + (j ^ (i & swz_m<A>()))];
+}
+
+// ---------------------------------------------------------------------------
+// One image as the tile operators see it.
+// ---------------------------------------------------------------------------
+struct ImgArgs {
+  const void* in;
+  void* out;
+  int64_t ldi, ldo;   // row pitches, in elements of the input / output type
+  uint32_t H, W;      // input rows / columns
+  uint32_t nTx, nTy;  // tiles per tile-row / per tile-column
+  uint64_t* vcarry;   // vertical carry words of this image's tile 0
+  uint64_t* hcarry;   // horizontal carry words of this image's tile 0
+  uint32_t vin, vout; // whole-chunk vector loads / stores allowed
+};
+
+// ---------------------------------------------------------------------------
+// Launch-state header at the start of every workspace. A zero-filled
+// workspace is a valid initial state.
+// ---------------------------------------------------------------------------
+struct WsHeader {
+  uint32_t counter;    // tile tickets handed out in the running launch
+  uint32_t done;       // workgroups that finished in the running launch
+  uint32_t epoch;      // tag of the running launch (0 is read as 1)
+  uint32_t reserved0;
+  uint64_t hwm_words;  // largest carry area (64-bit words) any launch used
+  uint64_t reserved[5];
+};
+constexpr uint32_t EPOCH_MAX = (1u << 30) - 1;
+constexpr int64_t WS_HEADER_BYTES = 256;
+
+__device__ __forceinline__ uint32_t take_ticket(WsHeader* h) {
+  return __hip_atomic_fetch_add(&h->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t read_epoch(WsHeader* h) {
+  const uint32_t e = __hip_atomic_load(&h->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return e == 0 ? 1u : e;
+}
+
+// Called by every workgroup after its last tile. The workgroup that finishes
+// last resets the ticket counters and advances the epoch; when the epoch would
+// wrap it clears every carry word any launch has used so that no stale tag can
+// be mistaken for a fresh one. Returns through the whole workgroup.
+__device__ __forceinline__ void finish_launch(WsHeader* h, uint64_t* carries, uint64_t words_used,
+                                              uint32_t* s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t d = __hip_atomic_fetch_add(&h->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t act = 0;
+    if (d == gridDim.x - 1) {
+      const uint32_t e = read_epoch(h);
+      act = (e >= EPOCH_MAX) ? 2u : 1u;
+    }
+    *s_flag = act;
+  }
+  __syncthreads();
+  const uint32_t act = *s_flag;
+  if (act == 0) return;
+  if (act == 2) {
+    uint64_t hwm = h->hwm_words;
+    if (words_used > hwm) hwm = words_used;
+    for (uint64_t i = threadIdx.x; i < hwm; i += blockDim.x) st_word(carries + i, 0);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t e = read_epoch(h);
+    const uint64_t hwm = h->hwm_words;
+    h->hwm_words = (act == 2) ? 0 : (words_used > hwm ? words_used : hwm);
+    h->counter = 0;
+    h->done = 0;
+    __hip_atomic_store(&h->epoch, act == 2 ? 1u : e + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+}  // namespace satdev

@@ -1,0 +1,74 @@
+// Host-side internal interface between the C ABI (sat_api.cpp) and the
+// translation units that instantiate the kernels (k_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/sat/sat.h"
+#include "sat_device.hpp"
+
+namespace sathost {
+
+// Pass / kernel families.
+enum Family : int {
+  FAM_SAT = 0,       // one-pass 2-D SAT (TileOp MODE_SAT)
+  FAM_SAT_REV = 1,   // one-pass reverse SAT (TileOp MODE_SAT, mirrored)
+  FAM_ROWS = 2,      // row scan, same layout (TileOp MODE_ROW)
+  FAM_COLS = 3,      // column scan (TileOp MODE_COL)
+  FAM_ROWS_T = 4,    // row scan + transposed store (TileOp MODE_ROW + TOUT)
+  FAM_SERIAL_T = 5,  // transpose + serial row scan (SerialTileOp)
+  FAM_COUNT = 6
+};
+
+// Tile geometry of a family for an output element size (bytes).
+struct Geometry {
+  int th, tw;            // tile rows / columns (of the input)
+  int64_t vwords, hwords;  // carry words per tile
+  int threads;
+};
+Geometry geometry(int family, size_t out_size);
+
+// One launch of a family over `planes` images of identical shape.
+struct Launch {
+  int family;
+  int scan;
+  int tin, tout;
+  const void* in;
+  void* out;
+  int64_t ldi, ldo;
+  int64_t in_plane, out_plane;  // plane strides (elements)
+  uint32_t width, height, planes;
+};
+
+size_t carry_bytes(const Launch& L);
+sat_status launch_family(const Launch& L, void* ws, size_t ws_bytes, hipStream_t stream);
+
+// Batched persistent kernel (k_batch.hip).
+struct BatchItem {
+  satdev::ImgArgs g;
+  uint32_t kind;       // dtype pair id (pair_id)
+  uint32_t tile_base;  // first global tile of this image
+};
+int pair_id(int tin, int tout);  // -1 if unsupported
+sat_status launch_batch(int out_size, const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles,
+                        void* ws, uint64_t* carries, uint64_t words_used, int grid, hipStream_t stream);
+Geometry batch_geometry(size_t out_size);
+int batch_threads(size_t out_size);
+int batch_blocks_per_cu(size_t out_size);
+
+// Box filter (k_box.hip).
+sat_status launch_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width,
+                           uint32_t height, uint32_t planes, const int32_t* boxes, uint32_t nbox, void* out,
+                           hipStream_t stream);
+sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x,
+                           int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
+                           void* ws, size_t ws_bytes, hipStream_t stream);
+size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes);
+
+// Misc.
+int device_cu_count();
+size_t dtype_size(int dt);
+
+}  // namespace sathost

@@ -1,0 +1,114 @@
+// Host-side launch templates shared by the kernel translation units.
+#pragma once
+
+#include "sat_host.hpp"
+#include "sat_tile.hpp"
+
+namespace sathost {
+
+using satdev::arith_t;
+
+// Default tile shapes: 4 waves; 16 rows per wave for 4-byte outputs (64x256
+// tiles, 64 accumulators per lane), 8 rows per wave for 8-byte outputs.
+template <typename TDst> constexpr int kRPT = sizeof(arith_t<TDst>) == 4 ? 16 : 8;
+template <typename TDst> constexpr int kSerialCW = sizeof(arith_t<TDst>) == 4 ? 64 : 32;
+constexpr int kNW = 4;
+
+template <class Op>
+inline Geometry geom_of() {
+  Geometry g;
+  g.th = Op::TH;
+  g.tw = Op::TW;
+  g.vwords = Op::VWORDS;
+  g.hwords = Op::HWORDS;
+  g.threads = Op::NT;
+  return g;
+}
+
+inline bool aligned_for(const void* p, int64_t pitch, int64_t plane, size_t esize) {
+  const size_t need = esize * 4 >= 16 ? 16 : esize * 4;
+  const uintptr_t a = (uintptr_t)p;
+  return (a % need == 0) && (((uint64_t)pitch * esize) % need == 0) && (((uint64_t)plane * esize) % need == 0);
+}
+
+template <class Op>
+inline uint64_t op_carry_words(const Launch& L) {
+  const uint64_t nTx = (L.width + Op::TW - 1) / Op::TW, nTy = (L.height + Op::TH - 1) / Op::TH;
+  const uint64_t ntiles = nTx * nTy * L.planes;
+  return ntiles * (uint64_t)(Op::VWORDS + Op::HWORDS);
+}
+
+template <class Op>
+inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStream_t stream) {
+  using TSrc = typename Op::Src;
+  using TDst = typename Op::Dst;
+  const uint64_t nTx = (L.width + Op::TW - 1) / Op::TW, nTy = (L.height + Op::TH - 1) / Op::TH;
+  const uint64_t tpp = nTx * nTy, ntiles = tpp * L.planes;
+  if (ntiles == 0) return SAT_OK;
+  if (ntiles > 0x7fffffffull) return SAT_ERR_INVALID;
+  const uint64_t vw = ntiles * Op::VWORDS, hw = ntiles * Op::HWORDS;
+  const uint64_t need = satdev::WS_HEADER_BYTES + (vw + hw) * 8;
+  if (ws == nullptr) return SAT_ERR_NULL;
+  if (ws_bytes < need) return SAT_ERR_WORKSPACE;
+  uint64_t* carries = (uint64_t*)((char*)ws + satdev::WS_HEADER_BYTES);
+  satdev::ImgArgs g;
+  g.in = L.in;
+  g.out = L.out;
+  g.ldi = L.ldi;
+  g.ldo = L.ldo;
+  g.H = L.height;
+  g.W = L.width;
+  g.nTx = (uint32_t)nTx;
+  g.nTy = (uint32_t)nTy;
+  g.vcarry = carries;
+  g.hcarry = carries + vw;
+  g.vin = aligned_for(L.in, L.ldi, L.in_plane, sizeof(TSrc)) ? 1u : 0u;
+  g.vout = aligned_for(L.out, L.ldo, L.out_plane, sizeof(TDst)) ? 1u : 0u;
+  hipLaunchKernelGGL(satdev::tiles_kernel<Op>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, g, (uint32_t)tpp,
+                     L.in_plane, L.out_plane, (satdev::WsHeader*)ws, carries, vw + hw);
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
+
+// The reference dispatchers' dtype pairs (ScanRow-BRLT.cu:274-376) plus u32->u32.
+// X(tin_enum, tout_enum, TSrc, TDst, float_out)
+#define SAT_FOR_EACH_PAIR(X)                  \
+  X(SAT_U8, SAT_U32, uint8_t, uint32_t, 0)    \
+  X(SAT_U8, SAT_F32, uint8_t, float, 1)       \
+  X(SAT_U8, SAT_F64, uint8_t, double, 1)      \
+  X(SAT_I8, SAT_I32, int8_t, int32_t, 0)      \
+  X(SAT_I8, SAT_F32, int8_t, float, 1)        \
+  X(SAT_I8, SAT_F64, int8_t, double, 1)       \
+  X(SAT_I32, SAT_I32, int32_t, int32_t, 0)    \
+  X(SAT_I32, SAT_F64, int32_t, double, 1)     \
+  X(SAT_F32, SAT_F32, float, float, 1)        \
+  X(SAT_F32, SAT_F64, float, double, 1)       \
+  X(SAT_F64, SAT_F64, double, double, 1)      \
+  X(SAT_U32, SAT_U32, uint32_t, uint32_t, 0)
+
+// Dispatch F<TSrc, TDst, SCAN>::go(L, ws, bytes, stream) on (L.tin, L.tout,
+// L.scan). Integer outputs are order-independent, so for them only the
+// Kogge-Stone order is instantiated.
+template <bool FLOAT_OUT> struct ScanPick;
+template <> struct ScanPick<true> {
+  template <template <typename, typename, int> class F, typename S, typename D>
+  static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) {
+    return L.scan == SAT_SCAN_LF ? F<S, D, 1>::go(L, ws, b, st) : F<S, D, 0>::go(L, ws, b, st);
+  }
+};
+template <> struct ScanPick<false> {
+  template <template <typename, typename, int> class F, typename S, typename D>
+  static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) {
+    return F<S, D, 0>::go(L, ws, b, st);
+  }
+};
+
+template <template <typename, typename, int> class F>
+inline sat_status dispatch_pairs(const Launch& L, void* ws, size_t b, hipStream_t st) {
+#define SAT_X(a, b_, S, D, fl) \
+  if (L.tin == a && L.tout == b_) return ScanPick<fl != 0>::template go<F, S, D>(L, ws, b, st);
+  SAT_FOR_EACH_PAIR(SAT_X)
+#undef SAT_X
+  return SAT_ERR_DTYPE;
+}
+
+}  // namespace sathost
