@@ -1,0 +1,211 @@
+"""Benchmark: summed-area-table throughput on one MI355X.
+
+Default workload: one-pass SAT of an f32 8192 x 8192 image (values U[0,1)),
+inputs resident in HBM. A step is one SAT of the image. Throughput is
+algorithmic bytes (H*W*(sizeof(in) + sizeof(out))) per second; the roofline is
+MI355X's 8 TB/s HBM peak. The reference's CPU integral (the oracle's
+restatement of IntegralImageParallel / IntegralImageSerial) is timed on the
+host cores of the same machine as the CPU baseline.
+
+Prints ONE JSON line. ``--gpus`` above 1 is not supported (one GPU only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_launches(fn, steps: int, warmup: int):
+    """Average ms per call over `steps` back-to-back calls on the current stream."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def time_each(fn, steps: int):
+    """Median of per-call event timings (ms)."""
+    import torch
+
+    ts = []
+    for _ in range(steps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        fn()
+        e.record()
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def cpu_baseline(x, tout, budget_s: float = 12.0):
+    """Mean of the reference's 6-run protocol (GetAvgTime, integralImage.h:199-211)
+    for the OpenMP path on all host cores and the serial path on one core."""
+    from oracle import oracle as O
+
+    res = {}
+    ncores = os.cpu_count() or 1
+    O.set_threads(ncores)
+    for name, fn, threads in (("parallel", O.parallel, ncores), ("serial", O.serial, 1)):
+        fn(x, tout)  # warm-up
+        reps = 6
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(x, tout)
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = (dt, threads)
+        if dt * reps > budget_s:
+            break
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the SAT path here has no multi-GPU mode "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import sat_amd
+    from oracle import oracle as O
+
+    dev = torch.device("cuda:0")
+    H = W = 8192
+    log(f"[bench] f32 {H}x{W} one-pass SAT: generating input")
+    x_np = O.gen("f32", (H, W), 20260101)
+    x = torch.from_numpy(x_np).to(dev)
+    y = torch.empty_like(x)
+
+    def step():
+        sat_amd.integral(x, alg=sat_amd.SINGLE_PASS, out=y)
+
+    step()
+    torch.cuda.synchronize()
+    # parity of the measured output (not timed)
+    log("[bench] checking the output against the FP64 oracle")
+    ref = O.sat_f64(x_np)
+    got = y.cpu().numpy().astype(np.float64)
+    rel = float(np.max(np.abs(got - ref) / np.maximum(ref, 1e-30)))
+    del got, ref
+
+    log(f"[bench] timing {args.steps} steps after {args.warmup} warm-up steps")
+    ms = time_launches(step, args.steps, args.warmup)
+    algo_bytes = H * W * (4 + 4)
+    gbs = algo_bytes / (ms * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_extra:
+        extra = extra_configs(O, sat_amd, dev)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle restatement of the reference CPU integral)")
+        r = cpu_baseline(x_np, np.float32)
+        dt, cores = r["parallel"]
+        cpu = {"value": algo_bytes / dt / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
+               "sample": f"IntegralImageParallel f32 {H}x{W}, OpenMP on all host cores, mean of 6 runs"}
+        if "serial" in r:
+            dts, _ = r["serial"]
+            cpu["serial_1core_GBs"] = algo_bytes / dts / 1e9
+
+    out = {
+        "metric": "SAT GB/s (in+out bytes) & % HBM peak",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (splitmix64 U[0,1) pixels)",
+        "config": {"workload": "one-pass SAT, f32 8192x8192, row-major, inputs resident in HBM",
+                   "height": H, "width": W, "algorithm": "single-pass 2-D decoupled look-back",
+                   "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
+                   "f32_max_rel_err_vs_fp64": rel},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if extra:
+        out["extra"] = extra
+    print(json.dumps(out))
+
+
+def extra_configs(O, sat_amd, dev):
+    """Secondary workloads (each median-timed): the other algorithms, dtypes and shapes."""
+    import numpy as np
+    import torch
+
+    res = {}
+
+    def run(name, x, out_dtype, alg, in_b, out_b, steps=10):
+        y = torch.empty(x.shape, dtype=out_dtype, device=dev)
+        f = lambda: sat_amd.integral(x, out_dtype=out_dtype, alg=alg, out=y)  # noqa: E731
+        f()
+        torch.cuda.synchronize()
+        ms = time_each(f, steps)
+        gbs = x.numel() * (in_b + out_b) / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "GBs": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 1)}
+        log(f"[bench] {name}: {ms:.4f} ms {gbs:.1f} GB/s")
+        return y
+
+    xf = torch.from_numpy(O.gen("f32", (8192, 8192), 20260101)).to(dev)
+    for alg in (0, 1, 2):
+        run(f"f32_8192_{sat_amd.ALG_NAMES[alg]}", xf, torch.float32, alg, 4, 4)
+    del xf
+    xu = torch.from_numpy(O.gen("u32_20", (8192, 8192), 20260102).view(np.int32)).to(dev).view(torch.uint32)
+    run("u32_8192_SinglePass", xu, torch.uint32, 3, 4, 4)
+    del xu
+    xd = torch.from_numpy(O.gen("f64n", (4096, 4096), 20260103)).to(dev)
+    run("f64_4096_SinglePass", xd, torch.float64, 3, 8, 8)
+    del xd
+    for h, w in ((65536, 256), (256, 65536)):
+        xt = torch.from_numpy(O.gen("f32", (h, w), 20260104)).to(dev)
+        run(f"f32_{h}x{w}_SinglePass", xt, torch.float32, 3, 4, 4)
+        del xt
+    x8 = torch.from_numpy(O.gen("u8", (16384, 16384), 20260105)).to(dev)
+    run("u8u32_16384_SinglePass", x8, torch.uint32, 3, 1, 4)
+    del x8
+    x2 = torch.from_numpy(O.gen("u8", (2048, 2048), 20260106)).to(dev)
+    for alg in (0, 1, 2, 3):
+        run(f"u8u32_2048_{sat_amd.ALG_NAMES[alg]}", x2, torch.uint32, alg, 1, 4)
+    torch.cuda.empty_cache()
+    return res
+
+
+if __name__ == "__main__":
+    main()

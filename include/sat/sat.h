@@ -187,6 +187,15 @@ int sat_batch_run(void* workspace, int ctas_per_cu, void* stream);
 int sat_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height,
                  uint32_t planes, const int32_t* boxes, uint32_t nbox, void* out, void* stream);
 
+/* Adjoint of sat_box_sums with respect to the SAT: adds +g at (y1,x1) and
+ * (y0-1,x0-1) and -g at (y0-1,x1) and (y1,x0-1) of each box (indices < 0
+ * dropped) into grad_sat, which the caller initialises (usually to zero).
+ * Coinciding corners accumulate with atomics, so f32 sums at a pixel hit by
+ * several corners may differ in the last bits from run to run. */
+int sat_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_sat,
+                    int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
+                    void* stream);
+
 /* Gradient of sum_b g[p*nbox+b] * boxsum_b(SAT(x_p)) with respect to x:
  * grad_x[p][y][x] = sum over boxes b of plane p containing (y, x) of g[p*nbox+b].
  * Computed as the reverse SAT of the boxes' corner impulses without

@@ -30,6 +30,22 @@ __global__ void __launch_bounds__(256) box_sums_kernel(const T* __restrict__ sat
   out[i] = ((s11 - s01) - s10) + s00;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) box_scatter_kernel(const int4* __restrict__ boxes, const T* __restrict__ gout,
+                                                          uint64_t count, uint32_t nbox, T* gs, int64_t pitch,
+                                                          int64_t plane_stride, uint32_t W, uint32_t H) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int4 b = boxes[i];
+  if (!box_ok(b, W, H)) return;
+  const T g = gout[i];
+  T* s = gs + (i / nbox) * plane_stride;
+  atomicAdd(&s[(int64_t)b.z * pitch + b.w], g);
+  if (b.x > 0) atomicAdd(&s[(int64_t)(b.x - 1) * pitch + b.w], -g);
+  if (b.y > 0) atomicAdd(&s[(int64_t)b.z * pitch + (b.y - 1)], -g);
+  if (b.x > 0 && b.y > 0) atomicAdd(&s[(int64_t)(b.x - 1) * pitch + (b.y - 1)], g);
+}
+
 template <typename T> struct GradCfg;
 template <> struct GradCfg<float> { static constexpr int NW = 4, RPT = 8; };
 template <> struct GradCfg<double> { static constexpr int NW = 4, RPT = 4; };
@@ -115,6 +131,24 @@ sat_status launch_box_sums(int dtype, const void* sat, int64_t pitch, int64_t pl
   else if (dtype == SAT_F64)
     hipLaunchKernelGGL(box_sums_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream, (const double*)sat,
                        pitch, plane_stride, width, height, count, nbox, (const int4*)boxes, (double*)out);
+  else
+    return SAT_ERR_DTYPE;
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
+
+sat_status launch_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_sat,
+                              int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
+                              hipStream_t stream) {
+  const uint64_t count = (uint64_t)planes * nbox;
+  if (count == 0) return SAT_OK;
+  const uint64_t blocks = (count + 255) / 256;
+  if (blocks > 0x7fffffffull) return SAT_ERR_INVALID;
+  if (dtype == SAT_F32)
+    hipLaunchKernelGGL(box_scatter_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream, (const int4*)boxes,
+                       (const float*)grad_out, count, nbox, (float*)grad_sat, pitch, plane_stride, width, height);
+  else if (dtype == SAT_F64)
+    hipLaunchKernelGGL(box_scatter_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream, (const int4*)boxes,
+                       (const double*)grad_out, count, nbox, (double*)grad_sat, pitch, plane_stride, width, height);
   else
     return SAT_ERR_DTYPE;
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;

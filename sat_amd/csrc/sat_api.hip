@@ -512,6 +512,18 @@ int sat_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride
                          (hipStream_t)stream);
 }
 
+int sat_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_sat,
+                    int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
+                    void* stream) {
+  if (dtype != SAT_F32 && dtype != SAT_F64) return SAT_ERR_DTYPE;
+  if ((uint64_t)planes * nbox == 0 || width == 0 || height == 0) return SAT_OK;
+  if (!boxes || !grad_out || !grad_sat) return SAT_ERR_NULL;
+  if (pitch < (int64_t)width) return SAT_ERR_INVALID;
+  if (planes > 1 && plane_stride < pitch * (int64_t)height) return SAT_ERR_INVALID;
+  return launch_box_scatter(dtype, boxes, nbox, grad_out, grad_sat, pitch, plane_stride, width, height, planes,
+                            (hipStream_t)stream);
+}
+
 size_t sat_box_grad_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes, uint32_t nbox) {
   (void)nbox;
   if (dtype != SAT_F32 && dtype != SAT_F64) return 0;

@@ -351,17 +351,18 @@ __device__ __forceinline__ void finish_launch(WsHeader* h, uint64_t* carries, ui
   const uint32_t act = *s_flag;
   if (act == 0) return;
   if (act == 2) {
-    uint64_t hwm = h->hwm_words;
+    uint64_t hwm = __hip_atomic_load(&h->hwm_words, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (words_used > hwm) hwm = words_used;
     for (uint64_t i = threadIdx.x; i < hwm; i += blockDim.x) st_word(carries + i, 0);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     const uint32_t e = read_epoch(h);
-    const uint64_t hwm = h->hwm_words;
-    h->hwm_words = (act == 2) ? 0 : (words_used > hwm ? words_used : hwm);
-    h->counter = 0;
-    h->done = 0;
+    const uint64_t hwm = __hip_atomic_load(&h->hwm_words, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&h->hwm_words, (act == 2) ? 0 : (words_used > hwm ? words_used : hwm), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&h->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&h->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&h->epoch, act == 2 ? 1u : e + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }

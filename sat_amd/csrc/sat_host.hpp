@@ -62,6 +62,9 @@ int batch_blocks_per_cu(size_t out_size);
 sat_status launch_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width,
                            uint32_t height, uint32_t planes, const int32_t* boxes, uint32_t nbox, void* out,
                            hipStream_t stream);
+sat_status launch_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_sat,
+                              int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
+                              hipStream_t stream);
 sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x,
                            int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
                            void* ws, size_t ws_bytes, hipStream_t stream);

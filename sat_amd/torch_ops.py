@@ -1,0 +1,298 @@
+"""Tensor-level entry points and autograd Functions over the C ABI.
+
+Tensors must live on a HIP device; pointers and the current stream are handed
+to ``libsat_amd.so``. Leading dimensions of (..., H, W) tensors are treated as
+independent planes (e.g. the N*C planes of an NCHW batch).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import torch
+
+from . import (BRLT_SCANROW, F32, F64, I8, I32, KOGGE_STONE, SCANROW_BRLT, SCANROW_COLUMN, SINGLE_PASS, U8, U32,
+               SatError, check, lib, sat_image)
+
+__all__ = [
+    "sat_dtype", "default_out_dtype", "integral", "integral_reverse", "block_scan", "seriel_scan", "scan_x", "scan_y",
+    "box_sums", "box_scatter", "box_grad", "BatchPlan", "sat", "box_sum", "box_filter", "SATFunction",
+    "BoxSumFunction", "BoxFilterFunction", "SCANROW_BRLT", "BRLT_SCANROW", "SCANROW_COLUMN", "SINGLE_PASS",
+]
+
+_TORCH2SAT = {torch.uint8: U8, torch.int8: I8, torch.int32: I32, torch.float32: F32, torch.float64: F64}
+if hasattr(torch, "uint32"):
+    _TORCH2SAT[torch.uint32] = U32
+_DEFAULT_OUT = {torch.uint8: torch.uint32 if hasattr(torch, "uint32") else torch.int32, torch.int8: torch.int32,
+                torch.int32: torch.int32, torch.float32: torch.float32, torch.float64: torch.float64}
+if hasattr(torch, "uint32"):
+    _DEFAULT_OUT[torch.uint32] = torch.uint32
+
+
+def sat_dtype(dt: torch.dtype) -> int:
+    try:
+        return _TORCH2SAT[dt]
+    except KeyError:
+        raise SatError(f"dtype {dt} is not supported") from None
+
+
+def default_out_dtype(dt: torch.dtype) -> torch.dtype:
+    return _DEFAULT_OUT[dt]
+
+
+def _stream(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _dev_check(*ts: torch.Tensor) -> None:
+    for t in ts:
+        if not t.is_cuda:
+            raise SatError("sat_amd operates on HIP device tensors only")
+
+
+def _planes(x: torch.Tensor):
+    """View (..., H, W) as (P, H, W) with unit column stride."""
+    if x.dim() < 2:
+        raise SatError("expected a tensor of at least 2 dimensions")
+    h, w = x.shape[-2], x.shape[-1]
+    x3 = x.reshape(-1, h, w)
+    if x3.stride(-1) != 1 or (x3.shape[0] > 1 and x3.stride(0) < x3.stride(1) * h):
+        x3 = x3.contiguous()
+    return x3
+
+
+def integral(x: torch.Tensor, out_dtype: torch.dtype | None = None, alg: int = SINGLE_PASS, scan: int = KOGGE_STONE,
+             out: torch.Tensor | None = None) -> torch.Tensor:
+    """Inclusive 2-D prefix sum over the last two dimensions (IntegralImageSerial,
+    integralImage.h:105-124). ``alg`` 0..2 run the reference's two-pass
+    compositions, 3 (default) the one-pass kernel."""
+    _dev_check(x)
+    out_dtype = out_dtype or default_out_dtype(x.dtype)
+    x3 = _planes(x)
+    p, h, w = x3.shape
+    if out is None:
+        out = torch.empty(x.shape, dtype=out_dtype, device=x.device)
+    if out.dtype != out_dtype or out.shape != x.shape or not out.is_contiguous():
+        raise SatError("`out` must be a contiguous tensor of the input's shape and the output dtype")
+    o3 = out.view(-1, h, w)
+    if x3.numel() == 0:
+        return out
+    st = lib().sat_integral_planes(alg, scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x3.data_ptr(), x3.stride(1),
+                                   x3.stride(0), o3.data_ptr(), o3.stride(1), o3.stride(0), w, h, p, None, 0,
+                                   _stream(x))
+    check(st, "sat_integral_planes")
+    return out
+
+
+def integral_reverse(x: torch.Tensor, scan: int = KOGGE_STONE) -> torch.Tensor:
+    """Suffix sums over the last two dimensions: the adjoint of :func:`integral`."""
+    _dev_check(x)
+    x3 = _planes(x)
+    p, h, w = x3.shape
+    out = torch.empty(x.shape, dtype=x.dtype, device=x.device)
+    if x3.numel() == 0:
+        return out
+    o3 = out.view(-1, h, w)
+    st = lib().sat_integral_reverse_planes(scan, sat_dtype(x.dtype), x3.data_ptr(), x3.stride(1), x3.stride(0),
+                                           o3.data_ptr(), o3.stride(1), o3.stride(0), w, h, p, None, 0, _stream(x))
+    check(st, "sat_integral_reverse_planes")
+    return out
+
+
+def _rows2d(x: torch.Tensor) -> torch.Tensor:
+    _dev_check(x)
+    if x.dim() != 2 or x.stride(1) != 1:
+        raise SatError("expected a 2-D tensor with unit column stride")
+    return x
+
+
+def block_scan(x: torch.Tensor, out_dtype: torch.dtype | None = None, scan: int = KOGGE_STONE,
+               out: torch.Tensor | None = None) -> torch.Tensor:
+    """BlockScan::blockScan (ScanRow-BRLT.cu:24-25): out[x][y] = sum_{k<=x} in[y][k]."""
+    x = _rows2d(x)
+    h, w = x.shape
+    out_dtype = out_dtype or default_out_dtype(x.dtype)
+    out = torch.empty((w, h), dtype=out_dtype, device=x.device) if out is None else out
+    check(lib().sat_block_scan(scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), out.data_ptr(), w,
+                               x.stride(0), h, out.stride(0), _stream(x)), "sat_block_scan")
+    return out
+
+
+def seriel_scan(x: torch.Tensor, out_dtype: torch.dtype | None = None, out: torch.Tensor | None = None) -> torch.Tensor:
+    """SerielScan::serielScan (BRLT-Scan.cu:14-15): same result as block_scan."""
+    x = _rows2d(x)
+    h, w = x.shape
+    out_dtype = out_dtype or default_out_dtype(x.dtype)
+    out = torch.empty((w, h), dtype=out_dtype, device=x.device) if out is None else out
+    check(lib().sat_seriel_scan(sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), out.data_ptr(), w,
+                                x.stride(0), h, out.stride(0), _stream(x)), "sat_seriel_scan")
+    return out
+
+
+def scan_x(x: torch.Tensor, out_dtype: torch.dtype | None = None, scan: int = KOGGE_STONE,
+           out: torch.Tensor | None = None) -> torch.Tensor:
+    """IncrementScan::ScanX (ScanRowColumn.cu:144-145): row prefix sums, same layout."""
+    x = _rows2d(x)
+    h, w = x.shape
+    out_dtype = out_dtype or default_out_dtype(x.dtype)
+    out = torch.empty((h, w), dtype=out_dtype, device=x.device) if out is None else out
+    check(lib().sat_scan_rows(scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), x.stride(0),
+                              out.data_ptr(), out.stride(0), w, h, _stream(x)), "sat_scan_rows")
+    return out
+
+
+def scan_y(x: torch.Tensor, out_dtype: torch.dtype | None = None, scan: int = KOGGE_STONE,
+           out: torch.Tensor | None = None) -> torch.Tensor:
+    """IncrementScan::ScanY (ScanRowColumn.cu:15-16): column prefix sums."""
+    x = _rows2d(x)
+    h, w = x.shape
+    out_dtype = out_dtype or default_out_dtype(x.dtype)
+    out = torch.empty((h, w), dtype=out_dtype, device=x.device) if out is None else out
+    check(lib().sat_scan_cols(scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), x.stride(0),
+                              out.data_ptr(), out.stride(0), w, h, _stream(x)), "sat_scan_cols")
+    return out
+
+
+def _boxes(boxes: torch.Tensor, planes: int) -> tuple[torch.Tensor, int]:
+    if boxes.dtype != torch.int32 or boxes.shape[-1] != 4:
+        raise SatError("boxes must be int32 (..., 4) records {y0, x0, y1, x1}")
+    b = boxes.reshape(planes, -1, 4).contiguous()
+    return b, b.shape[1]
+
+
+def box_sums(S: torch.Tensor, boxes: torch.Tensor) -> torch.Tensor:
+    """Box sums from SAT planes S (..., H, W) (f32/f64); boxes (P, nbox, 4) int32,
+    inclusive corners. Returns (P, nbox)."""
+    _dev_check(S, boxes)
+    s3 = _planes(S)
+    p, h, w = s3.shape
+    b, nbox = _boxes(boxes, p)
+    out = torch.empty((p, nbox), dtype=S.dtype, device=S.device)
+    check(lib().sat_box_sums(sat_dtype(S.dtype), s3.data_ptr(), s3.stride(1), s3.stride(0), w, h, p, b.data_ptr(),
+                             nbox, out.data_ptr(), _stream(S)), "sat_box_sums")
+    return out
+
+
+def box_scatter(boxes: torch.Tensor, g: torch.Tensor, height: int, width: int) -> torch.Tensor:
+    """Adjoint of :func:`box_sums` w.r.t. S: the corner impulses of every box."""
+    _dev_check(boxes, g)
+    p = g.shape[0]
+    b, nbox = _boxes(boxes, p)
+    g = g.reshape(p, nbox).contiguous()
+    out = torch.zeros((p, height, width), dtype=g.dtype, device=g.device)
+    check(lib().sat_box_scatter(sat_dtype(g.dtype), b.data_ptr(), nbox, g.data_ptr(), out.data_ptr(), width,
+                                width * height, width, height, p, _stream(g)), "sat_box_scatter")
+    return out
+
+
+def box_grad(boxes: torch.Tensor, g: torch.Tensor, height: int, width: int) -> torch.Tensor:
+    """grad of sum(g * box_sums(integral(x), boxes)) w.r.t. x, in one fused pass."""
+    _dev_check(boxes, g)
+    p = g.shape[0]
+    b, nbox = _boxes(boxes, p)
+    g = g.reshape(p, nbox).contiguous()
+    out = torch.empty((p, height, width), dtype=g.dtype, device=g.device)
+    check(lib().sat_box_grad(sat_dtype(g.dtype), b.data_ptr(), nbox, g.data_ptr(), out.data_ptr(), width,
+                             width * height, width, height, p, None, 0, _stream(g)), "sat_box_grad")
+    return out
+
+
+class BatchPlan:
+    """A prepared batch of (input, output) image pairs of uneven sizes, run by
+    one persistent launch. Keep the tensors alive while the plan is in use."""
+
+    def __init__(self, pairs: Sequence[tuple[torch.Tensor, torch.Tensor]]):
+        if not pairs:
+            raise SatError("empty batch")
+        self.pairs = list(pairs)
+        arr = (sat_image * len(self.pairs))()
+        for i, (x, y) in enumerate(self.pairs):
+            _dev_check(x, y)
+            if x.dim() != 2 or y.shape != x.shape or x.stride(1) != 1 or y.stride(1) != 1:
+                raise SatError("batch images must be 2-D, same shape in and out, unit column stride")
+            arr[i].in_ = x.data_ptr()
+            arr[i].out = y.data_ptr()
+            arr[i].in_pitch = x.stride(0)
+            arr[i].out_pitch = y.stride(0)
+            arr[i].width = x.shape[1]
+            arr[i].height = x.shape[0]
+            arr[i].tin = sat_dtype(x.dtype)
+            arr[i].tout = sat_dtype(y.dtype)
+        self._arr = arr
+        dev = self.pairs[0][0].device
+        nbytes = lib().sat_batch_workspace_bytes(C.cast(arr, C.c_void_p), len(self.pairs))
+        if nbytes == 0:
+            raise SatError("invalid batch (dtype pairs, sizes or pitches)")
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.device = dev
+        check(lib().sat_batch_prepare(C.cast(arr, C.c_void_p), len(self.pairs), self.workspace.data_ptr(), nbytes,
+                                      torch.cuda.current_stream(dev).cuda_stream), "sat_batch_prepare")
+
+    def run(self, ctas_per_cu: int = 0) -> None:
+        check(lib().sat_batch_run(self.workspace.data_ptr(), ctas_per_cu,
+                                  torch.cuda.current_stream(self.device).cuda_stream), "sat_batch_run")
+
+
+# ---------------------------------------------------------------------------
+# autograd
+# ---------------------------------------------------------------------------
+class SATFunction(torch.autograd.Function):
+    """y = SAT(x); the adjoint of a 2-D inclusive prefix sum is the 2-D suffix
+    sum, so grad_x = reverse SAT(grad_y)."""
+
+    @staticmethod
+    def forward(ctx, x, scan=KOGGE_STONE):
+        ctx.scan = scan
+        return integral(x, scan=scan)
+
+    @staticmethod
+    def backward(ctx, gy):
+        return integral_reverse(gy.contiguous(), scan=ctx.scan), None
+
+
+class BoxSumFunction(torch.autograd.Function):
+    """b = box_sums(S, boxes); grad_S = corner impulses of grad_b."""
+
+    @staticmethod
+    def forward(ctx, S, boxes):
+        ctx.save_for_backward(boxes)
+        ctx.hw = (S.shape[-2], S.shape[-1])
+        ctx.sshape = S.shape
+        return box_sums(S, boxes)
+
+    @staticmethod
+    def backward(ctx, gb):
+        (boxes,) = ctx.saved_tensors
+        gS = box_scatter(boxes, gb.contiguous(), *ctx.hw)
+        return gS.reshape(ctx.sshape), None
+
+
+class BoxFilterFunction(torch.autograd.Function):
+    """b = box_sums(SAT(x), boxes) with a fused backward: grad_x is the reverse
+    SAT of the corner impulses, built tile by tile from the box list."""
+
+    @staticmethod
+    def forward(ctx, x, boxes):
+        S = integral(x)
+        ctx.save_for_backward(boxes)
+        ctx.xshape = x.shape
+        return box_sums(S, boxes)
+
+    @staticmethod
+    def backward(ctx, gb):
+        (boxes,) = ctx.saved_tensors
+        h, w = ctx.xshape[-2], ctx.xshape[-1]
+        gx = box_grad(boxes, gb.contiguous(), h, w)
+        return gx.reshape(ctx.xshape), None
+
+
+def sat(x: torch.Tensor, scan: int = KOGGE_STONE) -> torch.Tensor:
+    return SATFunction.apply(x, scan)
+
+
+def box_sum(S: torch.Tensor, boxes: torch.Tensor) -> torch.Tensor:
+    return BoxSumFunction.apply(S, boxes)
+
+
+def box_filter(x: torch.Tensor, boxes: torch.Tensor) -> torch.Tensor:
+    return BoxFilterFunction.apply(x, boxes)

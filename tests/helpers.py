@@ -1,0 +1,41 @@
+"""Shared helpers of the parity tests (CPU side only: comparisons and inputs)."""
+from __future__ import annotations
+
+import numpy as np
+
+# generator kind per input dtype for random parity cases
+KIND = {"u8": "u8", "i8": "i8", "u32": "u32_20", "i32": "i32_16", "f32": "f32", "f64": "f64n"}
+INT_OUT = {"u32", "i32"}
+
+# f32 tolerance: |gpu - exact| <= F32_K * 2^-24 * SAT(|x|)(y, x) + tiny, i.e. a
+# small multiple of one rounding of the largest partial sum on the way. The
+# reference's own f32 CPU left fold has an error bound that grows with the
+# image size instead; see test_float_error_vs_cpu_reference.
+F32_K = 32.0
+F64_K = 32.0
+
+
+def sat_abs_f64(O, x: np.ndarray) -> np.ndarray:
+    return O.serial(np.abs(x.astype(np.float64)), np.float64)
+
+
+def assert_sat_close(O, x: np.ndarray, got: np.ndarray, tout: str, exact: np.ndarray | None = None,
+                     k: float | None = None) -> float:
+    """Integer outputs: bit-exact against the oracle. Float outputs: within the
+    stated bound of the FP64 oracle. Returns the max error in units of
+    u * SAT(|x|) (0 for integers)."""
+    if tout in INT_OUT:
+        if exact is None:
+            exact = O.serial(x, np.dtype({"u32": np.uint32, "i32": np.int32}[tout]))
+        bad = np.argwhere(got != exact)
+        assert bad.size == 0, f"{len(bad)} mismatches, first at {tuple(bad[0])}: {got[tuple(bad[0])]} != {exact[tuple(bad[0])]}"
+        return 0.0
+    ref = O.sat_f64(x) if exact is None else exact
+    scale = sat_abs_f64(O, x)
+    u = 2.0 ** -24 if tout == "f32" else 2.0 ** -53
+    k = k if k is not None else (F32_K if tout == "f32" else F64_K)
+    err = np.abs(got.astype(np.float64) - ref)
+    bound = k * u * scale + 1e-300
+    ratio = float(np.max(err / (u * scale + 1e-300))) if err.size else 0.0
+    assert np.all(err <= bound), f"max error {ratio:.2f} u*SAT(|x|) exceeds {k}"
+    return ratio

@@ -1,0 +1,146 @@
+"""GPU parity of the full SAT and the pass-level drop-ins against the oracle.
+
+Integer outputs must be bit-exact (including modulo-2^32 wraparound); float
+outputs must lie within the bound of tests/helpers.py of the FP64 oracle.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import INT_OUT, KIND, assert_sat_close
+
+pytestmark = pytest.mark.gpu
+
+NP = {"u8": np.uint8, "i8": np.int8, "u32": np.uint32, "i32": np.int32, "f32": np.float32, "f64": np.float64}
+TD = {"u8": torch.uint8, "i8": torch.int8, "u32": torch.uint32, "i32": torch.int32, "f32": torch.float32,
+      "f64": torch.float64}
+PAIRS = [("u8", "u32"), ("u8", "f32"), ("u8", "f64"), ("i8", "i32"), ("i8", "f32"), ("i8", "f64"), ("i32", "i32"),
+         ("i32", "f64"), ("f32", "f32"), ("f32", "f64"), ("f64", "f64"), ("u32", "u32")]
+SMALL = [(1, 1), (3, 5), (64, 256), (65, 257), (31, 300), (200, 17), (130, 520)]
+
+
+def to_dev(x: np.ndarray, dev) -> torch.Tensor:
+    if x.dtype == np.uint32:
+        return torch.from_numpy(x.view(np.int32)).to(dev).view(torch.uint32)
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def to_np(t: torch.Tensor) -> np.ndarray:
+    if t.dtype == torch.uint32:
+        return t.view(torch.int32).cpu().numpy().view(np.uint32)
+    return t.cpu().numpy()
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def sa():
+    import sat_amd
+
+    sat_amd.lib()
+    return sat_amd
+
+
+@pytest.mark.parametrize("alg", [0, 1, 2, 3])
+@pytest.mark.parametrize("tin,tout", PAIRS)
+def test_integral_small(O, dev, sa, alg, tin, tout):
+    say(f"[case] integral alg={alg} {tin}->{tout}")
+    for k, (h, w) in enumerate(SMALL):
+        x = O.gen(KIND[tin], (h, w), 1000 + 17 * k + alg)
+        y = sa.integral(to_dev(x, dev), out_dtype=TD[tout], alg=alg)
+        assert_sat_close(O, x, to_np(y), tout)
+
+
+@pytest.mark.parametrize("alg", [0, 1, 2, 3])
+@pytest.mark.parametrize("scan", [0, 1])
+def test_scan_orders_float(O, dev, sa, alg, scan):
+    say(f"[case] scan order alg={alg} scan={scan}")
+    x = O.gen("f32", (300, 700), 77)
+    y = sa.integral(to_dev(x, dev), alg=alg, scan=scan)
+    assert_sat_close(O, x, to_np(y), "f32")
+
+
+def test_all_ones_closed_form(O, dev, sa):
+    """The authors' known answer (integralImage.h:273): all-ones -> (x+1)(y+1)."""
+    h, w = 513, 1031
+    x = np.ones((h, w), np.uint8)
+    exp = ((np.arange(h, dtype=np.uint64)[:, None] + 1) * (np.arange(w, dtype=np.uint64)[None, :] + 1)).astype(np.uint32)
+    for alg in range(4):
+        say(f"[case] all-ones alg={alg}")
+        y = to_np(sa.integral(to_dev(x, dev), alg=alg))
+        assert np.array_equal(y, exp)
+
+
+def test_pitched_buffers(O, dev, sa):
+    """Row pitches larger than the width, odd element offsets (no vector path)."""
+    h, w = 150, 333
+    big = O.gen("f32", (h, w + 29), 5)
+    x = to_dev(big, dev)[:, 3:3 + w]
+    assert x.stride(0) == w + 29
+    for alg in range(4):
+        say(f"[case] pitched alg={alg}")
+        y = sa.integral(x, alg=alg)
+        assert_sat_close(O, big[:, 3:3 + w], to_np(y), "f32")
+
+
+@pytest.mark.parametrize("tin,tout", PAIRS)
+def test_pass_kernels(O, dev, sa, tin, tout):
+    say(f"[case] passes {tin}->{tout}")
+    ntout = NP[tout]
+    for k, (h, w) in enumerate([(1, 1), (70, 300), (257, 65), (64, 512)]):
+        x = O.gen(KIND[tin], (h, w), 300 + k)
+        xd = to_dev(x, dev)
+        rows = O.allrow(x, ntout)
+        cols = O.colscan(x, ntout)
+        rows_t = O.transpose(rows)
+        for name, got, exp in (
+            ("scan_x", sa.scan_x(xd, out_dtype=TD[tout]), rows),
+            ("scan_y", sa.scan_y(xd, out_dtype=TD[tout]), cols),
+            ("block_scan", sa.block_scan(xd, out_dtype=TD[tout]), rows_t),
+            ("seriel_scan", sa.seriel_scan(xd, out_dtype=TD[tout]), rows_t),
+        ):
+            g = to_np(got)
+            assert g.shape == exp.shape, name
+            if tout in INT_OUT:
+                assert np.array_equal(g, exp), f"{name} {h}x{w}"
+            else:
+                ref = (O.allrow(x, np.float64) if name != "scan_y" else O.colscan(x, np.float64))
+                if name in ("block_scan", "seriel_scan"):
+                    ref = O.transpose(ref)
+                absx = np.abs(x.astype(np.float64))
+                sc = O.allrow(absx, np.float64) if name != "scan_y" else O.colscan(absx, np.float64)
+                if name in ("block_scan", "seriel_scan"):
+                    sc = O.transpose(sc)
+                u = 2.0 ** -24 if tout == "f32" else 2.0 ** -53
+                err = np.abs(g.astype(np.float64) - ref)
+                assert np.all(err <= 32 * u * sc + 1e-300), f"{name} {h}x{w}"
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64", "u32"])
+def test_reverse(O, dev, sa, dt):
+    say(f"[case] reverse {dt}")
+    for k, (h, w) in enumerate([(1, 1), (77, 300), (64, 256), (300, 1000)]):
+        x = O.gen({"f32": "f32", "f64": "f64n", "u32": "u32_20"}[dt], (h, w), 900 + k)
+        y = to_np(sa.integral_reverse(to_dev(x, dev)))
+        if dt == "u32":
+            assert np.array_equal(y, O.reverse(x))
+        else:
+            ref = O.reverse(x.astype(np.float64))
+            sc = O.reverse(np.abs(x.astype(np.float64)))
+            u = 2.0 ** -24 if dt == "f32" else 2.0 ** -53
+            assert np.all(np.abs(y - ref) <= 32 * u * sc + 1e-300)
+
+
+def test_planes(O, dev, sa):
+    say("[case] planes")
+    x = O.gen("f32n", (2, 3, 100, 300), 4242)
+    y = to_np(sa.integral(to_dev(x, dev)))
+    for n in range(2):
+        for c in range(3):
+            assert_sat_close(O, x[n, c], y[n, c], "f32")
