@@ -1,9 +1,27 @@
 // One-pass SAT kernels (forward and reverse) and the family geometry table.
 #include "sat_launch.hpp"
 
+namespace satdev {
+
+__global__ void __launch_bounds__(256) clear_kernel(uint64_t* p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+}  // namespace satdev
+
 namespace sathost {
 
 using namespace satdev;
+
+sat_status launch_clear(void* p, size_t bytes, hipStream_t stream) {
+  const uint64_t n = bytes / 8;
+  if (n == 0) return SAT_OK;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(clear_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, (uint64_t*)p, n);
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
 
 template <typename S, typename D, int SC>
 struct FamSat {

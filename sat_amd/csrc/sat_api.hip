@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -63,20 +64,28 @@ int device_cu_count() {
 }
 
 // ---------------------------------------------------------------------------
-// Library-managed workspaces, one per (device, stream); grown on demand.
+// Library-managed workspaces, one set per (device, stream), grown on demand.
+// Carry words are validated by their launch tag, which is only sound if the
+// carry buffer never holds anything but zeros or tagged words: the buffer
+// holding the launch header and carries is therefore kept apart from the
+// buffer holding two-pass temporaries (raw pixel sums could otherwise pass
+// for a tag). A caller-provided workspace holds both, so its header and carry
+// area are cleared by a fill kernel before every call.
 // ---------------------------------------------------------------------------
 struct ManagedWs {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
 static std::mutex g_ws_mu;
-static std::map<std::pair<int, uintptr_t>, ManagedWs> g_ws;
+static std::map<std::tuple<int, uintptr_t, int>, ManagedWs> g_ws;
 
-static sat_status managed_ws(hipStream_t st, size_t need, void** out) {
+enum WsKind : int { WS_CARRY = 0, WS_TMP = 1 };
+
+static sat_status managed_buf(hipStream_t st, int kind, size_t need, void** out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return SAT_ERR_HIP;
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  ManagedWs& m = g_ws[{dev, (uintptr_t)st}];
+  ManagedWs& m = g_ws[std::make_tuple(dev, (uintptr_t)st, kind)];
   if (m.bytes < need) {
     if (m.ptr) {
       if (hipStreamSynchronize(st) != hipSuccess) return SAT_ERR_HIP;
@@ -88,7 +97,7 @@ static sat_status managed_ws(hipStream_t st, size_t need, void** out) {
     nb = (nb + 0xffff) & ~(size_t)0xffff;
     void* p = nullptr;
     if (hipMalloc(&p, nb) != hipSuccess) return SAT_ERR_HIP;
-    if (hipMemsetAsync(p, 0, nb, st) != hipSuccess) {
+    if (kind == WS_CARRY && launch_clear(p, nb, st) != SAT_OK) {
       (void)hipFree(p);
       return SAT_ERR_HIP;
     }
@@ -117,9 +126,13 @@ static sat_status check_2d(const void* in, const void* out, int64_t ldi, int64_t
 static sat_status run_with_ws(const Launch& L, void* ws, size_t ws_bytes, hipStream_t st) {
   const size_t need = carry_bytes(L);
   if (!ws) {
-    const sat_status s = managed_ws(st, need, &ws);
+    const sat_status s = managed_buf(st, WS_CARRY, need, &ws);
     if (s != SAT_OK) return s;
     ws_bytes = need;
+  } else {
+    if (ws_bytes < need) return SAT_ERR_WORKSPACE;
+    const sat_status s = launch_clear(ws, need, st);
+    if (s != SAT_OK) return s;
   }
   return launch_family(L, ws, ws_bytes, st);
 }
@@ -227,14 +240,21 @@ static sat_status plan_integral(int alg, int scan, int tin, int tout, const void
 }
 
 static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_t st) {
+  void* tmp = nullptr;
   if (!ws) {
-    const sat_status s = managed_ws(st, P.total, &ws);
+    sat_status s = managed_buf(st, WS_CARRY, P.carry, &ws);
     if (s != SAT_OK) return s;
-    ws_bytes = P.total;
+    if (P.p2.family >= 0) {
+      s = managed_buf(st, WS_TMP, P.total - P.tmp_off, &tmp);
+      if (s != SAT_OK) return s;
+    }
+  } else {
+    if (ws_bytes < P.total) return SAT_ERR_WORKSPACE;
+    const sat_status s = launch_clear(ws, P.carry, st);
+    if (s != SAT_OK) return s;
+    tmp = (char*)ws + P.tmp_off;
   }
-  if (ws_bytes < P.total) return SAT_ERR_WORKSPACE;
-  if (P.p2.family < 0) return launch_family(P.p1, ws, ws_bytes, st);
-  void* tmp = (char*)ws + P.tmp_off;
+  if (P.p2.family < 0) return launch_family(P.p1, ws, P.carry, st);
   P.p1.out = tmp;
   P.p2.in = tmp;
   sat_status s = launch_family(P.p1, ws, P.carry, st);
@@ -466,7 +486,7 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
     it.g.hcarry = carries + (uint64_t)info.total_tiles * g.vwords + (uint64_t)it.tile_base * g.hwords;
   }
   hipStream_t st = (hipStream_t)stream;
-  if (hipMemsetAsync(workspace, 0, info.items_off, st) != hipSuccess) return SAT_ERR_HIP;
+  if (launch_clear(workspace, info.items_off, st) != SAT_OK) return SAT_ERR_HIP;
   if (!items.empty() &&
       hipMemcpyAsync((char*)workspace + info.items_off, items.data(), sizeof(BatchItem) * items.size(),
                      hipMemcpyHostToDevice, st) != hipSuccess)
@@ -541,9 +561,13 @@ int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* gra
   hipStream_t st = (hipStream_t)stream;
   const size_t need = box_grad_carry_bytes(dtype, width, height, planes);
   if (!workspace) {
-    const sat_status s = managed_ws(st, need, &workspace);
+    const sat_status s = managed_buf(st, WS_CARRY, need, &workspace);
     if (s != SAT_OK) return s;
     workspace_bytes = need;
+  } else {
+    if (workspace_bytes < need) return SAT_ERR_WORKSPACE;
+    const sat_status s = launch_clear(workspace, need, st);
+    if (s != SAT_OK) return s;
   }
   return launch_box_grad(dtype, boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, workspace,
                          workspace_bytes, st);

@@ -70,6 +70,9 @@ sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const
                            void* ws, size_t ws_bytes, hipStream_t stream);
 size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes);
 
+// Zero `bytes` (rounded down to 8) at `p` with a fill kernel (k_sat.hip).
+sat_status launch_clear(void* p, size_t bytes, hipStream_t stream);
+
 // Misc.
 int device_cu_count();
 size_t dtype_size(int dt);
