@@ -77,6 +77,14 @@ int sat_abi_version(void);
 /* Size in bytes of one element of `dtype`, 0 if unknown. */
 size_t sat_dtype_size(int dtype);
 
+/* Reports and clears the fault word of a workspace (NULL: the
+ * library-managed carry workspace of the calling device and `stream`): 1 if
+ * some look-back since the last query gave up waiting for a carry (its
+ * results are then invalid), 0 if none did, a negative sat_status on failure.
+ * Synchronises `stream`. A launch never spins forever: a look-back whose
+ * carry does not arrive within about a second gives up and raises the fault. */
+int sat_workspace_fault(void* workspace, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Pass-level drop-ins. Pitches (`widthStride`, `heightStride`) are in elements
  * of the pointed-to type, as GpuData::DataPitch() returns them

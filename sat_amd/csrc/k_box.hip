@@ -191,6 +191,7 @@ static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* g
   g.hcarry = carries + vw;
   g.vin = 0;
   g.vout = aligned_for(grad_x, pitch, plane_stride, sizeof(T)) ? 1u : 0u;
+  g.fault = &((WsHeader*)ws)->fault;
   hipLaunchKernelGGL(box_grad_kernel<T>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, (const int4*)boxes,
                      (const T*)grad_out, nbox, g, (uint32_t)tpp, plane_stride, (WsHeader*)ws, carries, vw + hw);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;

@@ -264,7 +264,7 @@ static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_
 
 // ---------------------------------------------------------------------------
 // Batch plans. Layout in the workspace:
-//   [0, 256)          launch-state header
+//   [0, 512)          launch-state header
 //   [256, items)      carry words
 //   items             BatchItem[count]
 //   tile_img          uint32_t[total_tiles]
@@ -322,6 +322,7 @@ static sat_status batch_layout(const sat_image* im, int count, BatchPlanInfo* in
     it.g.hcarry = nullptr;
     it.g.vin = 0;
     it.g.vout = 0;
+    it.g.fault = nullptr;  // set on the device address in sat_batch_prepare
     {
       const size_t si = dtype_size(im[i].tin), so = dtype_size(im[i].tout);
       const size_t ni = si * 4 >= 16 ? 16 : si * 4, no = so * 4 >= 16 ? 16 : so * 4;
@@ -360,6 +361,27 @@ const char* sat_status_string(int status) {
     case SAT_ERR_UNSUPPORTED: return "not supported by this build";
     default: return "unknown status";
   }
+}
+
+int sat_workspace_fault(void* workspace, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!workspace) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -SAT_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto f = g_ws.find(std::make_tuple(dev, (uintptr_t)st, (int)WS_CARRY));
+    if (f == g_ws.end()) return 0;
+    workspace = f->second.ptr;
+  }
+  uint32_t v = 0;
+  uint32_t* p = &((satdev::WsHeader*)workspace)->fault;
+  if (hipStreamSynchronize(st) != hipSuccess) return -SAT_ERR_HIP;
+  if (hipMemcpy(&v, p, sizeof v, hipMemcpyDeviceToHost) != hipSuccess) return -SAT_ERR_HIP;
+  if (v) {
+    const uint32_t z = 0;
+    if (hipMemcpy(p, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess) return -SAT_ERR_HIP;
+  }
+  return v ? 1 : 0;
 }
 
 int sat_block_scan(int scan, int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
@@ -482,6 +504,7 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
   const Geometry g = batch_geometry(info.out_size);
   uint64_t* carries = (uint64_t*)((char*)workspace + satdev::WS_HEADER_BYTES);
   for (auto& it : items) {
+    it.g.fault = &((satdev::WsHeader*)workspace)->fault;
     it.g.vcarry = carries + (uint64_t)it.tile_base * g.vwords;
     it.g.hcarry = carries + (uint64_t)info.total_tiles * g.vwords + (uint64_t)it.tile_base * g.hwords;
   }

@@ -196,26 +196,117 @@ template <typename A> struct Carry {
   }
 };
 
+// Polls of an unpublished carry word after which a look-back gives up (about
+// a second). A launch whose carries stop arriving (a corrupted workspace, a
+// workspace shared by overlapping calls) then ends instead of hanging the
+// GPU: the give-up is recorded in the workspace header's fault word, which
+// the host reads with sat_workspace_fault().
+constexpr uint32_t SPIN_LIMIT = 1u << 20;
+
 // Decoupled look-back over `n` predecessors, nearest first. `e` is the carry
 // entry of the nearest predecessor, `step` the entry distance to the next one.
-// Aggregates are summed until an inclusive prefix is found (or the chain ends).
-template <typename A>
-__device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t step, uint32_t n,
-                                       uint32_t ep4) {
+// Each round issues the loads of a window of D predecessors at once (one
+// memory round trip), then folds the longest published prefix of the window:
+// aggregates are summed until an inclusive prefix ends the walk. A window
+// whose nearest word is not yet published is retried after a short sleep.
+template <typename A, int D>
+__device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t step, uint32_t n, uint32_t ep4,
+                                       uint32_t* fault) {
   A acc = zero<A>();
+  uint32_t spins = 0;
   while (n > 0) {
-    A v;
-    const uint32_t st = Carry<A>::get(base, e, ep4, v);
-    if (st == 0) {
+    A v[D];
+    uint32_t st[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      st[d] = 0;
+      v[d] = zero<A>();
+      if ((uint32_t)d < n) st[d] = Carry<A>::get(base, e - (int64_t)d * step, ep4, v[d]);
+    }
+    uint32_t used = 0;
+    bool done = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (!done && used == (uint32_t)d && st[d] != 0) {
+        acc += v[d];
+        used = d + 1;
+        done = st[d] == ST_INC;
+      }
+    }
+    if (done) break;
+    if (used == 0) {
+      if (++spins >= SPIN_LIMIT) {
+        __hip_atomic_fetch_or(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    acc += v;
-    if (st == ST_INC) break;
-    e -= step;
-    --n;
+    e -= (int64_t)used * step;
+    n -= used;
   }
   return acc;
+}
+
+// K independent look-back chains per lane over the same predecessors (for
+// example two columns of one tile): every round issues the windows of all K
+// chains together, so the chains share their memory round trips.
+template <typename A, int D, int K>
+__device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t (&e0)[K], int64_t step, uint32_t n0,
+                                            uint32_t ep4, uint32_t* fault, A (&acc)[K]) {
+  int64_t e[K];
+  uint32_t n[K];
+  bool fin[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    acc[k] = zero<A>();
+    e[k] = e0[k];
+    n[k] = n0;
+    fin[k] = n0 == 0;
+  }
+  uint32_t spins = 0;
+  for (;;) {
+    bool all = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) all = all && fin[k];
+    if (all) break;
+    A v[K][D];
+    uint32_t st[K][D];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        st[k][d] = 0;
+        v[k][d] = zero<A>();
+        if (!fin[k] && (uint32_t)d < n[k]) st[k][d] = Carry<A>::get(base, e[k] - (int64_t)d * step, ep4, v[k][d]);
+      }
+    bool progress = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (fin[k]) continue;
+      uint32_t used = 0;
+      bool done = false;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (!done && used == (uint32_t)d && st[k][d] != 0) {
+          acc[k] += v[k][d];
+          used = d + 1;
+          done = st[k][d] == ST_INC;
+        }
+      }
+      if (used) progress = true;
+      e[k] -= (int64_t)used * step;
+      n[k] -= used;
+      if (done || n[k] == 0) fin[k] = true;
+    }
+    if (!progress) {
+      if (++spins >= SPIN_LIMIT) {
+        __hip_atomic_fetch_or(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -306,6 +397,7 @@ struct ImgArgs {
   uint64_t* vcarry;   // vertical carry words of this image's tile 0
   uint64_t* hcarry;   // horizontal carry words of this image's tile 0
   uint32_t vin, vout; // whole-chunk vector loads / stores allowed
+  uint32_t* fault;    // workspace fault word (look-back give-up)
 };
 
 // ---------------------------------------------------------------------------
@@ -313,15 +405,23 @@ struct ImgArgs {
 // workspace is a valid initial state.
 // ---------------------------------------------------------------------------
 struct WsHeader {
-  uint32_t counter;    // tile tickets handed out in the running launch
-  uint32_t done;       // workgroups that finished in the running launch
-  uint32_t epoch;      // tag of the running launch (0 is read as 1)
-  uint32_t reserved0;
+  // Each field the workgroups touch sits on its own 128-byte line: every
+  // workgroup performs one atomic on `counter` and one on `done`, and reads
+  // `epoch`; sharing a line makes the reads queue behind the atomics.
+  uint32_t counter;  // tile tickets handed out in the running launch
+  uint32_t pad0[31];
+  uint32_t done;     // workgroups that finished in the running launch
+  uint32_t pad1[31];
+  uint32_t epoch;    // tag of the running launch (0 is read as 1)
+  uint32_t pad2;
   uint64_t hwm_words;  // largest carry area (64-bit words) any launch used
-  uint64_t reserved[5];
+  uint64_t pad3[14];
+  uint32_t fault;      // nonzero once a look-back gave up (sticky until cleared by the host)
+  uint32_t pad4[31];
 };
 constexpr uint32_t EPOCH_MAX = (1u << 30) - 1;
-constexpr int64_t WS_HEADER_BYTES = 256;
+constexpr int64_t WS_HEADER_BYTES = 512;
+static_assert(sizeof(WsHeader) <= WS_HEADER_BYTES, "header size");
 
 __device__ __forceinline__ uint32_t take_ticket(WsHeader* h) {
   return __hip_atomic_fetch_add(&h->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -339,7 +439,9 @@ __device__ __forceinline__ void finish_launch(WsHeader* h, uint64_t* carries, ui
                                               uint32_t* s_flag) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t d = __hip_atomic_fetch_add(&h->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    // relaxed: the count only elects the workgroup that resets the header; an
+    // acquire/release here would write back and invalidate L2 per workgroup
+    const uint32_t d = __hip_atomic_fetch_add(&h->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t act = 0;
     if (d == gridDim.x - 1) {
       const uint32_t e = read_epoch(h);

@@ -3,14 +3,22 @@
 // A tile is TH rows x 256 columns of the input. A workgroup of NW waves loads
 // it with one coalesced 4-element access per lane and row (wave w holds rows
 // [w*RPT, (w+1)*RPT), lane l holds columns [4l, 4l+4)), scans it in registers,
-// and joins its neighbours through decoupled look-back on tagged carry words:
+// and joins its neighbours through decoupled look-back on tagged carry words.
+// For the 2-D SAT the value at tile-local (i, j) of tile (I, J) is
 //
-//   vertical carries   one word per tile column: the tile's column sums (AGG)
-//                      and the column sums of everything down to and including
-//                      the tile (INC);
-//   horizontal carries one word per tile row: the sum of the row prefix of the
-//                      tile's rows restricted to the tile's columns (AGG) and
-//                      the row's SAT value at the tile's right edge (INC).
+//   S = L(i,j) + sum_{j'<=j} Cab(j') + sum_{i'<=i} Rab(i') + Corner
+//
+// with L the tile-local SAT, Cab(j) the column sums above the tile, Rab(i)
+// the row sums left of the tile (rows of this tile row only) and Corner the
+// sum of everything above and to the left. The three carries come from three
+// independent look-backs, so none waits for another:
+//
+//   vertical words   per tile column: the tile's column sums (AGG), the column
+//                    sums down to and including the tile (INC); plus one
+//                    corner slot: the tile's Sum_i Rab(i) (AGG), and the corner
+//                    of the tile below (INC);
+//   horizontal words per tile row: the tile's row sums (AGG), the row sums up
+//                    to and including the tile (INC).
 //
 // Tiles are numbered by tickets taken from a counter when a workgroup starts,
 // row-major over the tile grid, so every predecessor a tile waits for has
@@ -22,6 +30,11 @@
 namespace satdev {
 
 enum : int { MODE_ROW = 1, MODE_COL = 2, MODE_SAT = 3 };
+
+// predecessors whose carry words one look-back round reads at once (a lane
+// that follows two column chains reads LB_WINDOW_V per chain)
+constexpr int LB_WINDOW = 8;
+constexpr int LB_WINDOW_V = 4;
 
 template <bool C, typename T, typename F> struct cond { using type = T; };
 template <typename T, typename F> struct cond<false, T, F> { using type = F; };
@@ -43,18 +56,32 @@ struct TileOp {
   static constexpr int NT = 64 * NW;
   static constexpr bool DO_ROW = (MODE & MODE_ROW) != 0;
   static constexpr bool DO_COL = (MODE & MODE_COL) != 0;
-  static constexpr int CPT_V = (TW + NT - 1) / NT;
-  static constexpr int64_t VWORDS = DO_COL ? (int64_t)TW * Carry<A>::WORDS : 0;  // per tile
+  static constexpr bool CORNER = DO_ROW && DO_COL;
+  // vertical look-backs run on the first NVW waves (all waves for a column
+  // pass), the horizontal and corner look-backs on the last wave
+  static constexpr int NVW = !DO_COL ? 1 : (DO_ROW ? NW / 2 : NW);
+  static constexpr int CPV = TW / (64 * NVW);  // columns per vertical lane
+  static constexpr int HWAVE = NW - 1;
+  static constexpr int64_t VS = TW + (CORNER ? 1 : 0);  // vertical entries per tile
+  static constexpr int64_t VWORDS = DO_COL ? VS * Carry<A>::WORDS : 0;  // per tile
   static constexpr int64_t HWORDS = DO_ROW ? (int64_t)TH * Carry<A>::WORDS : 0;  // per tile
   static_assert(!(TOUT && (REV || DO_COL)), "transposed store is a row-scan pass only");
-  static_assert(RPT <= 64, "one lane per tile row in the horizontal phase");
+  static_assert(TH <= 64, "one lane per tile row in the horizontal look-back");
+  static_assert(!CORNER || NW >= 2, "the 2-D pass splits look-backs over waves");
+  static_assert(TW % (64 * NVW) == 0, "vertical lanes");
 
   struct SmemCarry {
-    A ct[DO_COL ? NW : 1][TW];  // per-wave column totals
+    A ct[DO_COL ? NW : 1][TW];  // per-wave column sums
     A cab[DO_COL ? TW : 1];     // column sums above the tile
+    A rs[DO_ROW ? TH : 1];      // row sums of the tile
+    A rc[DO_ROW ? TH : 1];      // row carry of each tile row
+    A corner;
   };
   struct SmemStage {
     A st[TH * TW];  // XOR-swizzled, see stage_put4 / stage_get
+    A rs[TH];
+    A rc[TH];
+    A corner;
   };
   using Smem = typename cond<TOUT, SmemStage, SmemCarry>::type;
 
@@ -82,19 +109,42 @@ struct TileOp {
     return k.py0 + (REV ? TH - 1 - li : li);
   }
 
+  // A tile that lies wholly inside the image, with aligned rows, is loaded
+  // and stored with straight-line vector accesses, so every lane keeps all
+  // RPT row accesses in flight at once; edge tiles take the bounds-checked
+  // path. The test is uniform over the workgroup.
+  __device__ static __forceinline__ bool interior(const ImgArgs& g, const Coords& k) {
+    return k.py0 + TH <= (int64_t)g.H && k.px0 + TW <= (int64_t)g.W;
+  }
+
   __device__ static __forceinline__ void run(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm) {
     const Coords k = coords(g, t);
     const TSrc* __restrict__ in = (const TSrc*)g.in;
     const bool vin = g.vin != 0;
     // ---- load (each wave: RPT coalesced row segments of 256 elements) ----
     A v[RPT][4];
+    if (vin && interior(g, k)) {
+      typename Vec4<TSrc>::type q[RPT];
+      const TSrc* p0 = in + prow_of(k, 0) * g.ldi + k.pcol;
+      const int64_t step = REV ? -g.ldi : g.ldi;
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      const int64_t prow = prow_of(k, r);
-      A x[4];
-      load4<TSrc, TDst>(in + prow * g.ldi, k.pcol, g.W, prow < (int64_t)g.H, vin, x);
+      for (int r = 0; r < RPT; ++r) q[r] = *(const typename Vec4<TSrc>::type*)(p0 + r * step);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
+      for (int r = 0; r < RPT; ++r) {
+        const A x[4] = {cvt_in<TSrc, TDst>(q[r].x), cvt_in<TSrc, TDst>(q[r].y), cvt_in<TSrc, TDst>(q[r].z),
+                        cvt_in<TSrc, TDst>(q[r].w)};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int64_t prow = prow_of(k, r);
+        A x[4];
+        load4<TSrc, TDst>(in + prow * g.ldi, k.pcol, g.W, prow < (int64_t)g.H, vin, x);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
+      }
     }
     finish(g, t, ep4, sm, k, v);
   }
@@ -108,31 +158,42 @@ struct TileOp {
     const int64_t py0 = k.py0, px0 = k.px0, pcol = k.pcol;
     const bool vout = g.vout != 0;
 
-    // ---- column prefix inside the wave; publish the tile's column sums ----
-    A colsum[CPT_V];
+    // ---- per-wave column sums; the tile's column sums are published first ----
+    A colsum[CPV];
     if constexpr (DO_COL) {
+      if constexpr (!DO_ROW) {
+        // column pass: the in-lane column prefix is the result within the wave
 #pragma unroll
-      for (int r = 1; r < RPT; ++r)
+        for (int r = 1; r < RPT; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) v[r][c] += v[r - 1][c];
+          for (int c = 0; c < 4; ++c) v[r][c] += v[r - 1][c];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sm.ct[w][4 * lane + c] = v[RPT - 1][c];
+        for (int c = 0; c < 4; ++c) sm.ct[w][4 * lane + c] = v[RPT - 1][c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          A sum = v[0][c];
+#pragma unroll
+          for (int r = 1; r < RPT; ++r) sum += v[r][c];
+          sm.ct[w][4 * lane + c] = sum;
+        }
+      }
       __syncthreads();
+      if (w < NVW) {
 #pragma unroll
-      for (int s = 0; s < CPT_V; ++s) {
-        const int j = tid + s * NT;
-        if (j < TW) {
+        for (int q = 0; q < CPV; ++q) {
+          const int j = q * 64 * NVW + tid;
           A cs = sm.ct[0][j];
 #pragma unroll
-          for (int q = 1; q < NW; ++q) cs += sm.ct[q][j];
-          colsum[s] = cs;
-          Carry<A>::put(g.vcarry, (int64_t)t * TW + j, cs, ep4 | (I == 0 ? ST_INC : ST_AGG));
+          for (int u = 1; u < NW; ++u) cs += sm.ct[u][j];
+          colsum[q] = cs;
+          Carry<A>::put(g.vcarry, (int64_t)t * VS + j, cs, ep4 | (I == 0 ? ST_INC : ST_AGG));
         }
       }
     }
 
-    // ---- row prefix: in-lane over 4 columns, then across the 64 lanes ----
-    A rowtot[RPT];
+    // ---- row prefix: in-lane over 4 columns, then across the 64 lanes; the
+    //      tile's row sums are published as soon as they are known ----
     if constexpr (DO_ROW) {
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
@@ -141,29 +202,82 @@ struct TileOp {
         v[r][3] += v[r][2];
         const A inc = wave_scan<SCAN>(v[r][3], lane);
         const A exc = wave_shr1(inc);
-        rowtot[r] = readlane(inc, 63);
+        const A rsum = readlane(inc, 63);
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[r][c] += exc;
-      }
-    }
-
-    // ---- vertical look-back; per-column carry into this wave's rows ----
-    A colAdd[4] = {zero<A>(), zero<A>(), zero<A>(), zero<A>()};
-    A Tw = zero<A>();
-    if constexpr (DO_COL) {
-#pragma unroll
-      for (int s = 0; s < CPT_V; ++s) {
-        const int j = tid + s * NT;
-        if (j < TW) {
-          A acc = zero<A>();
-          if (I > 0) {
-            acc = look_back<A>(g.vcarry, (int64_t)(t - g.nTx) * TW + j, (int64_t)g.nTx * TW, I, ep4);
-            Carry<A>::put(g.vcarry, (int64_t)t * TW + j, acc + colsum[s], ep4 | ST_INC);
-          }
-          sm.cab[j] = acc;
+        if (lane == r) {
+          const int i = w * RPT + r;
+          Carry<A>::put(g.hcarry, (int64_t)t * TH + i, rsum, ep4 | (J == 0 ? ST_INC : ST_AGG));
+          sm.rs[i] = rsum;
         }
       }
+      if constexpr (DO_COL) {
+        // tile-local column prefix of the row prefixes: the wave-local SAT
+#pragma unroll
+        for (int r = 1; r < RPT; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[r][c] += v[r - 1][c];
+      }
       __syncthreads();
+    }
+
+    // ---- look-backs: vertical on the first NVW waves, horizontal and corner
+    //      on the last wave; they run concurrently ----
+    if constexpr (DO_COL) {
+      if (w < NVW) {
+        A acc[CPV];
+#pragma unroll
+        for (int q = 0; q < CPV; ++q) acc[q] = zero<A>();
+        if (I > 0) {
+          int64_t e[CPV];
+#pragma unroll
+          for (int q = 0; q < CPV; ++q) e[q] = (int64_t)(t - g.nTx) * VS + q * 64 * NVW + tid;
+          look_back_k<A, LB_WINDOW_V, CPV>(g.vcarry, e, (int64_t)g.nTx * VS, I, ep4, g.fault, acc);
+#pragma unroll
+          for (int q = 0; q < CPV; ++q)
+            Carry<A>::put(g.vcarry, (int64_t)t * VS + q * 64 * NVW + tid, acc[q] + colsum[q], ep4 | ST_INC);
+        }
+#pragma unroll
+        for (int q = 0; q < CPV; ++q) sm.cab[q * 64 * NVW + tid] = acc[q];
+      }
+    }
+    if constexpr (DO_ROW) {
+      if (w == HWAVE) {
+        A rab = zero<A>();
+        if (lane < TH && J > 0) {
+          const int64_t e = (int64_t)t * TH + lane;
+          rab = look_back<A, LB_WINDOW>(g.hcarry, e - TH, TH, J, ep4, g.fault);
+          Carry<A>::put(g.hcarry, e, rab + sm.rs[lane], ep4 | ST_INC);
+        }
+        if constexpr (CORNER) {
+          // row carry = prefix of Rab over the tile's rows; its total feeds
+          // the corner slot of the tiles below
+          const A inc = wave_scan<SCAN>(rab, lane);
+          if (lane < TH) sm.rc[lane] = inc;
+          const A ksum = readlane(inc, 63);
+          if (lane == 0) {
+            const int64_t e = (int64_t)t * VS + TW;
+            A corner = zero<A>();
+            if (I == 0) {
+              Carry<A>::put(g.vcarry, e, ksum, ep4 | ST_INC);
+            } else {
+              Carry<A>::put(g.vcarry, e, ksum, ep4 | ST_AGG);
+              corner = look_back<A, LB_WINDOW>(g.vcarry, e - (int64_t)g.nTx * VS, (int64_t)g.nTx * VS, I, ep4,
+                                                 g.fault);
+              Carry<A>::put(g.vcarry, e, corner + ksum, ep4 | ST_INC);
+            }
+            sm.corner = corner;
+          }
+        } else {
+          if (lane < TH) sm.rc[lane] = rab;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- per-column and per-row additions ----
+    A colAdd[4] = {zero<A>(), zero<A>(), zero<A>(), zero<A>()};
+    if constexpr (DO_COL) {
       A U[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -178,7 +292,6 @@ struct TileOp {
         U[3] += U[2];
         const A inc = wave_scan<SCAN>(U[3], lane);
         const A exc = wave_shr1(inc);
-        Tw = readlane(inc, 63);
 #pragma unroll
         for (int c = 0; c < 4; ++c) colAdd[c] = U[c] + exc;
       } else {
@@ -186,32 +299,18 @@ struct TileOp {
         for (int c = 0; c < 4; ++c) colAdd[c] = U[c];
       }
     }
-
-    // ---- horizontal look-back: lane r of wave w owns tile row w*RPT + r ----
     A rp[RPT];
     if constexpr (DO_ROW) {
-      A my_a = zero<A>();
+      A corner = zero<A>();
+      if constexpr (CORNER) corner = sm.corner;
 #pragma unroll
-      for (int r = 0; r < RPT; ++r)
-        if (lane == r) my_a = rowtot[r] + Tw;
-      A my_rp = zero<A>();
-      if (lane < RPT) {
-        const int64_t e = (int64_t)t * TH + w * RPT + lane;
-        if (J == 0) {
-          Carry<A>::put(g.hcarry, e, my_a, ep4 | ST_INC);
-        } else {
-          Carry<A>::put(g.hcarry, e, my_a, ep4 | ST_AGG);
-          my_rp = look_back<A>(g.hcarry, e - TH, TH, J, ep4);
-          Carry<A>::put(g.hcarry, e, my_rp + my_a, ep4 | ST_INC);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RPT; ++r) rp[r] = readlane(my_rp, r);
+      for (int r = 0; r < RPT; ++r) rp[r] = sm.rc[w * RPT + r] + corner;
     }
 
     // ---- store ----
     if constexpr (!TOUT) {
       TDst* __restrict__ out = (TDst*)g.out;
+      const bool fast = vout && interior(g, k);
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int li = w * RPT + r;
@@ -224,7 +323,17 @@ struct TileOp {
           if constexpr (DO_ROW) s += rp[r];
           y[REV ? 3 - c : c] = s;
         }
-        store4<TDst>(out + prow * g.ldo, pcol, g.W, prow < (int64_t)g.H, vout, y);
+        if (fast) {
+          typename Vec4<TDst>::type q;
+          q.x = cvt_out<TDst>(y[0]);
+          q.y = cvt_out<TDst>(y[1]);
+          q.z = cvt_out<TDst>(y[2]);
+          q.w = cvt_out<TDst>(y[3]);
+          TDst* o0 = out + (py0 + (REV ? TH - 1 - w * RPT : w * RPT)) * g.ldo + pcol;
+          *(typename Vec4<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo)) = q;
+        } else {
+          store4<TDst>(out + prow * g.ldo, pcol, g.W, prow < (int64_t)g.H, vout, y);
+        }
       }
     } else {
       // Block-register-local transpose: stage the scanned tile in LDS, then
@@ -323,7 +432,7 @@ struct SerialTileOp {
         Carry<A>::put(g.hcarry, e, a, ep4 | ST_INC);
       } else {
         Carry<A>::put(g.hcarry, e, a, ep4 | ST_AGG);
-        rp = look_back<A>(g.hcarry, e - TH, TH, J, ep4);
+        rp = look_back<A, LB_WINDOW>(g.hcarry, e - TH, TH, J, ep4, g.fault);
         Carry<A>::put(g.hcarry, e, rp + a, ep4 | ST_INC);
       }
       sm.rp[lane] = rp;
@@ -347,8 +456,10 @@ struct SerialTileOp {
 // One tile per workgroup; the workgroup count equals the tile count of all
 // planes. Ticket t covers plane t / tpp, tile t % tpp of that plane.
 // ---------------------------------------------------------------------------
+// Four waves per SIMD (at most 128 registers per lane): the tile kernels are
+// latency-bound on their look-backs and need the resident tiles.
 template <class Op>
-__global__ void __launch_bounds__(Op::NT) tiles_kernel(ImgArgs g, uint32_t tpp, int64_t in_plane, int64_t out_plane,
+__global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(4))) tiles_kernel(ImgArgs g, uint32_t tpp, int64_t in_plane, int64_t out_plane,
                                                        WsHeader* hdr, uint64_t* carries, uint64_t words_used) {
   __shared__ typename Op::Smem sm;
   __shared__ uint32_t s_ticket, s_ep, s_flag;
