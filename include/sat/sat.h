@@ -160,8 +160,8 @@ int sat_integral_reverse_planes(int scan, int dtype, const void* in, int64_t in_
                                 void* stream);
 
 /* ---------------------------------------------------------------------------
- * Batched SAT of many images of uneven sizes in ONE persistent launch walking
- * a queue of (image, tile) items. No reference counterpart.
+ * Batched SAT of many images of uneven sizes in ONE launch walking a queue
+ * of (image, tile) items. No reference counterpart.
  * Images may mix dtype pairs whose outputs have the same width (all 4-byte
  * or all 8-byte outputs).
  * ------------------------------------------------------------------------- */
@@ -180,10 +180,10 @@ size_t sat_batch_workspace_bytes(const sat_image* images, int count);
 /* Validates the batch, builds the work queue on the host and copies it into
  * the workspace (blocking copy; call outside graph capture). */
 int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_t workspace_bytes, void* stream);
-/* One persistent launch over the prepared queue. Allocates nothing, copies
- * nothing: may be captured in a hipGraph and replayed. `ctas_per_cu` = 0
- * picks the tuned / default residency. */
-int sat_batch_run(void* workspace, int ctas_per_cu, void* stream);
+/* Runs the prepared queue: one workgroup per (image, tile) item in a single
+ * launch (preceded by a one-workgroup epoch bump). Allocates nothing, copies
+ * nothing: may be captured in a hipGraph and replayed. */
+int sat_batch_run(void* workspace, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Box-filter readout on summed-area tables (f32 or f64), P planes.

@@ -1,7 +1,7 @@
-// Batched SAT: one persistent launch walks a queue of (image, tile) items over
-// images of uneven sizes and mixed dtype pairs. Tickets are global tile
-// numbers, image-major and row-major inside an image, so the look-back of a
-// tile only ever waits for tiles of its own image handed out earlier.
+// Batched SAT: one launch walks a queue of (image, tile) items over images of
+// uneven sizes and mixed dtype pairs. Queue positions are image-major and
+// row-major inside an image, so the look-back of a tile only ever waits for
+// tiles of its own image that precede it in the queue.
 #include "sat_launch.hpp"
 
 namespace satdev {
@@ -22,51 +22,46 @@ __device__ __forceinline__ void run_kind(const ImgArgs& g, uint32_t lt, uint32_t
   Op::run(g, lt, ep4, *reinterpret_cast<typename Op::Smem*>(smem));
 }
 
+// One workgroup per queue item: item t = blockIdx.x is tile t - tile_base of
+// image tile_img[t]. The launch epoch is the workspace's device epoch,
+// advanced by bump_epoch_kernel before every launch, so the same launch can
+// be replayed from a captured graph.
 template <int OUTB>
-__global__ void __launch_bounds__(64 * BatchCfg<OUTB>::NW)
-    batch_kernel(WsHeader* hdr, const sathost::BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_img,
-                 uint32_t total, uint64_t* carries, uint64_t words_used) {
+__global__ void __launch_bounds__(64 * BatchCfg<OUTB>::NW) __attribute__((amdgpu_waves_per_eu(4)))
+    batch_kernel(const WsHeader* hdr, const sathost::BatchItem* __restrict__ items,
+                 const uint32_t* __restrict__ tile_img) {
   using Cfg = BatchCfg<OUTB>;
   using Ref = typename Cfg::Ref;
   using RefOp = TileOp<Ref, Ref, MODE_SAT, false, false, 0, Cfg::NW, Cfg::RPT>;
   __shared__ typename RefOp::Smem sm;
-  __shared__ uint32_t s_t, s_ep, s_flag;
-  if (threadIdx.x == 0) s_ep = read_epoch(hdr);
-  for (;;) {
-    if (threadIdx.x == 0) s_t = take_ticket(hdr);
-    __syncthreads();
-    const uint32_t t = s_t;
-    if (t >= total) break;
-    const uint32_t ep4 = s_ep << 2;
-    const uint32_t img = __builtin_amdgcn_readfirstlane(tile_img[t]);
-    const sathost::BatchItem& it = items[img];
-    const ImgArgs g = it.g;
-    const uint32_t lt = t - it.tile_base;
-    const uint32_t kind = __builtin_amdgcn_readfirstlane(it.kind);
-    if constexpr (OUTB == 4) {
-      switch (kind) {
-        case K_U8_U32: run_kind<4, uint8_t, uint32_t>(g, lt, ep4, &sm); break;
-        case K_U8_F32: run_kind<4, uint8_t, float>(g, lt, ep4, &sm); break;
-        case K_I8_I32: run_kind<4, int8_t, int32_t>(g, lt, ep4, &sm); break;
-        case K_I8_F32: run_kind<4, int8_t, float>(g, lt, ep4, &sm); break;
-        case K_I32_I32: run_kind<4, int32_t, int32_t>(g, lt, ep4, &sm); break;
-        case K_F32_F32: run_kind<4, float, float>(g, lt, ep4, &sm); break;
-        case K_U32_U32: run_kind<4, uint32_t, uint32_t>(g, lt, ep4, &sm); break;
-        default: break;
-      }
-    } else {
-      switch (kind) {
-        case K_U8_F64: run_kind<8, uint8_t, double>(g, lt, ep4, &sm); break;
-        case K_I8_F64: run_kind<8, int8_t, double>(g, lt, ep4, &sm); break;
-        case K_I32_F64: run_kind<8, int32_t, double>(g, lt, ep4, &sm); break;
-        case K_F32_F64: run_kind<8, float, double>(g, lt, ep4, &sm); break;
-        case K_F64_F64: run_kind<8, double, double>(g, lt, ep4, &sm); break;
-        default: break;
-      }
+  const uint32_t ep4 = launch_ep4(0, hdr);
+  const uint32_t t = blockIdx.x;
+  const uint32_t img = __builtin_amdgcn_readfirstlane(tile_img[t]);
+  const sathost::BatchItem& it = items[img];
+  const ImgArgs g = it.g;
+  const uint32_t lt = t - it.tile_base;
+  const uint32_t kind = __builtin_amdgcn_readfirstlane(it.kind);
+  if constexpr (OUTB == 4) {
+    switch (kind) {
+      case K_U8_U32: run_kind<4, uint8_t, uint32_t>(g, lt, ep4, &sm); break;
+      case K_U8_F32: run_kind<4, uint8_t, float>(g, lt, ep4, &sm); break;
+      case K_I8_I32: run_kind<4, int8_t, int32_t>(g, lt, ep4, &sm); break;
+      case K_I8_F32: run_kind<4, int8_t, float>(g, lt, ep4, &sm); break;
+      case K_I32_I32: run_kind<4, int32_t, int32_t>(g, lt, ep4, &sm); break;
+      case K_F32_F32: run_kind<4, float, float>(g, lt, ep4, &sm); break;
+      case K_U32_U32: run_kind<4, uint32_t, uint32_t>(g, lt, ep4, &sm); break;
+      default: break;
     }
-    __syncthreads();
+  } else {
+    switch (kind) {
+      case K_U8_F64: run_kind<8, uint8_t, double>(g, lt, ep4, &sm); break;
+      case K_I8_F64: run_kind<8, int8_t, double>(g, lt, ep4, &sm); break;
+      case K_I32_F64: run_kind<8, int32_t, double>(g, lt, ep4, &sm); break;
+      case K_F32_F64: run_kind<8, float, double>(g, lt, ep4, &sm); break;
+      case K_F64_F64: run_kind<8, double, double>(g, lt, ep4, &sm); break;
+      default: break;
+    }
   }
-  finish_launch(hdr, carries, words_used, &s_flag);
 }
 
 }  // namespace satdev
@@ -98,26 +93,15 @@ Geometry batch_geometry(size_t out_size) {
 
 int batch_threads(size_t out_size) { return 64 * (out_size == 8 ? satdev::BatchCfg<8>::NW : satdev::BatchCfg<4>::NW); }
 
-int batch_blocks_per_cu(size_t out_size) {
-  int n = 0;
-  hipError_t e;
+sat_status launch_batch(int out_size, const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles,
+                        void* ws, hipStream_t stream) {
+  if (total_tiles == 0) return SAT_OK;
   if (out_size == 8)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, satdev::batch_kernel<8>, batch_threads(8), 0);
+    hipLaunchKernelGGL(satdev::batch_kernel<8>, dim3(total_tiles), dim3(batch_threads(8)), 0, stream,
+                       (const satdev::WsHeader*)ws, items, tile_img);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, satdev::batch_kernel<4>, batch_threads(4), 0);
-  if (e != hipSuccess || n < 1) n = 1;
-  return n;
-}
-
-sat_status launch_batch(int out_size, const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles, void* ws,
-                        uint64_t* carries, uint64_t words_used, int grid, hipStream_t stream) {
-  if (grid < 1) grid = 1;
-  if (out_size == 8)
-    hipLaunchKernelGGL(satdev::batch_kernel<8>, dim3(grid), dim3(batch_threads(8)), 0, stream,
-                       (satdev::WsHeader*)ws, items, tile_img, total_tiles, carries, words_used);
-  else
-    hipLaunchKernelGGL(satdev::batch_kernel<4>, dim3(grid), dim3(batch_threads(4)), 0, stream,
-                       (satdev::WsHeader*)ws, items, tile_img, total_tiles, carries, words_used);
+    hipLaunchKernelGGL(satdev::batch_kernel<4>, dim3(total_tiles), dim3(batch_threads(4)), 0, stream,
+                       (const satdev::WsHeader*)ws, items, tile_img);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 

@@ -57,7 +57,7 @@ template <> struct GradCfg<double> { static constexpr int NW = 4, RPT = 4; };
 template <typename T>
 __global__ void __launch_bounds__(64 * GradCfg<T>::NW)
     box_grad_kernel(const int4* __restrict__ boxes, const T* __restrict__ gout, uint32_t nbox, ImgArgs g,
-                    uint32_t tpp, int64_t out_plane, WsHeader* hdr, uint64_t* carries, uint64_t words_used) {
+                    uint32_t tpp, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr) {
   using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
   constexpr int TH = Op::TH, TW = Op::TW, NT = Op::NT, RPT = GradCfg<T>::RPT;
   union Smem {
@@ -65,13 +65,8 @@ __global__ void __launch_bounds__(64 * GradCfg<T>::NW)
     T imp[TH * TW];
   };
   __shared__ Smem sm;
-  __shared__ uint32_t s_ticket, s_ep, s_flag;
-  if (threadIdx.x == 0) {
-    s_ticket = take_ticket(hdr);
-    s_ep = read_epoch(hdr);
-  }
-  __syncthreads();
-  const uint32_t t = s_ticket;
+  const uint32_t ep4 = launch_ep4(ep_arg, hdr);
+  const uint32_t t = blockIdx.x;
   const uint32_t p = t / tpp, lt = t - p * tpp;
   ImgArgs gp = g;
   gp.out = (T*)g.out + (int64_t)p * out_plane;
@@ -108,8 +103,7 @@ __global__ void __launch_bounds__(64 * GradCfg<T>::NW)
     for (int c = 0; c < 4; ++c) v[r][c] = sm.imp[ly * TW + lcol + 3 - c];
   }
   __syncthreads();
-  Op::finish(gp, lt, s_ep << 2, sm.c, k, v);
-  finish_launch(hdr, carries, words_used, &s_flag);
+  Op::finish(gp, lt, ep4, sm.c, k, v);
 }
 
 }  // namespace satdev
@@ -169,7 +163,7 @@ size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t
 template <typename T>
 static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
                               int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes, void* ws,
-                              size_t ws_bytes, hipStream_t stream) {
+                              size_t ws_bytes, uint32_t epoch, hipStream_t stream) {
   using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
   const uint64_t nTx = (width + Op::TW - 1) / Op::TW, nTy = (height + Op::TH - 1) / Op::TH;
   const uint64_t tpp = nTx * nTy, ntiles = tpp * planes;
@@ -193,19 +187,19 @@ static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* g
   g.vout = aligned_for(grad_x, pitch, plane_stride, sizeof(T)) ? 1u : 0u;
   g.fault = &((WsHeader*)ws)->fault;
   hipLaunchKernelGGL(box_grad_kernel<T>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, (const int4*)boxes,
-                     (const T*)grad_out, nbox, g, (uint32_t)tpp, plane_stride, (WsHeader*)ws, carries, vw + hw);
+                     (const T*)grad_out, nbox, g, (uint32_t)tpp, plane_stride, epoch, (const WsHeader*)ws);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 
 sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x,
                            int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
-                           void* ws, size_t ws_bytes, hipStream_t stream) {
+                           void* ws, size_t ws_bytes, uint32_t epoch, hipStream_t stream) {
   if (dtype == SAT_F32)
     return grad_launch<float>(boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, ws, ws_bytes,
-                              stream);
+                              epoch, stream);
   if (dtype == SAT_F64)
     return grad_launch<double>(boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, ws, ws_bytes,
-                               stream);
+                               epoch, stream);
   return SAT_ERR_DTYPE;
 }
 

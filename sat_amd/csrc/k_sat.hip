@@ -8,11 +8,33 @@ __global__ void __launch_bounds__(256) clear_kernel(uint64_t* p, uint64_t n) {
     p[i] = 0;
 }
 
+// Advances the device epoch of a workspace for the launch that follows on the
+// same stream; on wrap-around it clears `words` carry words first.
+__global__ void __launch_bounds__(256) bump_epoch_kernel(WsHeader* hdr, uint64_t* carries, uint64_t words) {
+  __shared__ uint32_t s_next;
+  if (threadIdx.x == 0) {
+    const uint32_t e = hdr->epoch;
+    s_next = (e < EPOCH_DEVICE_BASE || e >= EPOCH_MAX) ? EPOCH_DEVICE_BASE : e + 1;
+  }
+  __syncthreads();
+  const uint32_t next = s_next;
+  if (next == EPOCH_DEVICE_BASE)
+    for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) carries[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(&hdr->epoch, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace satdev
 
 namespace sathost {
 
 using namespace satdev;
+
+sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream) {
+  hipLaunchKernelGGL(bump_epoch_kernel, dim3(1), dim3(256), 0, stream, (WsHeader*)ws,
+                     (uint64_t*)((char*)ws + WS_HEADER_BYTES), carry_words);
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
 
 sat_status launch_clear(void* p, size_t bytes, hipStream_t stream) {
   const uint64_t n = bytes / 8;

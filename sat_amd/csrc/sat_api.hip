@@ -75,13 +75,14 @@ int device_cu_count() {
 struct ManagedWs {
   void* ptr = nullptr;
   size_t bytes = 0;
+  uint32_t epoch = 0;  // host epoch counter of a carry buffer
 };
 static std::mutex g_ws_mu;
 static std::map<std::tuple<int, uintptr_t, int>, ManagedWs> g_ws;
 
 enum WsKind : int { WS_CARRY = 0, WS_TMP = 1 };
 
-static sat_status managed_buf(hipStream_t st, int kind, size_t need, void** out) {
+static sat_status managed_buf(hipStream_t st, int kind, size_t need, void** out, ManagedWs** entry = nullptr) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return SAT_ERR_HIP;
   std::lock_guard<std::mutex> lk(g_ws_mu);
@@ -103,8 +104,65 @@ static sat_status managed_buf(hipStream_t st, int kind, size_t need, void** out)
     }
     m.ptr = p;
     m.bytes = nb;
+    m.epoch = 0;
   }
   *out = m.ptr;
+  if (entry) *entry = &m;
+  return SAT_OK;
+}
+
+// Carry area of one API call and the epochs of its launches.
+struct CarryCtx {
+  void* ws = nullptr;
+  size_t bytes = 0;        // usable bytes of the area (header + carries)
+  ManagedWs* managed = nullptr;
+  bool device_epoch = false;  // stream is being captured: bump on the device
+  uint32_t local = 0;         // cleared caller area: next epoch
+};
+
+static bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) return false;
+  return cs == hipStreamCaptureStatusActive;
+}
+
+static sat_status carry_open(hipStream_t st, void* user_ws, size_t user_bytes, size_t need, CarryCtx* c) {
+  if (user_ws) {
+    if (user_bytes < need) return SAT_ERR_WORKSPACE;
+    const sat_status s = launch_clear(user_ws, need, st);
+    if (s != SAT_OK) return s;
+    c->ws = user_ws;
+    c->bytes = need;
+    c->local = 1;
+    return SAT_OK;
+  }
+  const sat_status s = managed_buf(st, WS_CARRY, need, &c->ws, &c->managed);
+  if (s != SAT_OK) return s;
+  c->bytes = c->managed->bytes;
+  c->device_epoch = stream_capturing(st);
+  return SAT_OK;
+}
+
+// Epoch for the next launch on the call's carry area.
+static sat_status carry_epoch(CarryCtx& c, hipStream_t st, uint32_t* ep) {
+  if (c.local) {
+    *ep = c.local++;
+    return SAT_OK;
+  }
+  const uint64_t words = (c.bytes - satdev::WS_HEADER_BYTES) / 8;
+  if (c.device_epoch) {
+    *ep = 0;
+    return launch_bump_epoch(c.ws, words, st);
+  }
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  uint32_t e = ++c.managed->epoch;
+  if (e >= satdev::EPOCH_DEVICE_BASE) {
+    // host range exhausted: no stale tag may survive the restart at 1
+    const sat_status s = launch_clear(c.ws, c.bytes, st);
+    if (s != SAT_OK) return s;
+    e = c.managed->epoch = 1;
+  }
+  *ep = e;
   return SAT_OK;
 }
 
@@ -123,18 +181,13 @@ static sat_status check_2d(const void* in, const void* out, int64_t ldi, int64_t
   return SAT_OK;
 }
 
-static sat_status run_with_ws(const Launch& L, void* ws, size_t ws_bytes, hipStream_t st) {
-  const size_t need = carry_bytes(L);
-  if (!ws) {
-    const sat_status s = managed_buf(st, WS_CARRY, need, &ws);
-    if (s != SAT_OK) return s;
-    ws_bytes = need;
-  } else {
-    if (ws_bytes < need) return SAT_ERR_WORKSPACE;
-    const sat_status s = launch_clear(ws, need, st);
-    if (s != SAT_OK) return s;
-  }
-  return launch_family(L, ws, ws_bytes, st);
+static sat_status run_with_ws(Launch L, void* ws, size_t ws_bytes, hipStream_t st) {
+  CarryCtx c;
+  sat_status s = carry_open(st, ws, ws_bytes, carry_bytes(L), &c);
+  if (s != SAT_OK) return s;
+  s = carry_epoch(c, st, &L.epoch);
+  if (s != SAT_OK) return s;
+  return launch_family(L, c.ws, c.bytes, st);
 }
 
 static Launch make_launch(int family, int scan, int tin, int tout, const void* in, int64_t ldi, void* out, int64_t ldo,
@@ -153,6 +206,7 @@ static Launch make_launch(int family, int scan, int tin, int tout, const void* i
   L.width = width;
   L.height = height;
   L.planes = 1;
+  L.epoch = 0;
   return L;
 }
 
@@ -241,25 +295,23 @@ static sat_status plan_integral(int alg, int scan, int tin, int tout, const void
 
 static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_t st) {
   void* tmp = nullptr;
-  if (!ws) {
-    sat_status s = managed_buf(st, WS_CARRY, P.carry, &ws);
-    if (s != SAT_OK) return s;
-    if (P.p2.family >= 0) {
-      s = managed_buf(st, WS_TMP, P.total - P.tmp_off, &tmp);
-      if (s != SAT_OK) return s;
-    }
-  } else {
+  CarryCtx c;
+  if (ws) {
     if (ws_bytes < P.total) return SAT_ERR_WORKSPACE;
-    const sat_status s = launch_clear(ws, P.carry, st);
-    if (s != SAT_OK) return s;
     tmp = (char*)ws + P.tmp_off;
+  } else if (P.p2.family >= 0) {
+    const sat_status s = managed_buf(st, WS_TMP, P.total - P.tmp_off, &tmp);
+    if (s != SAT_OK) return s;
   }
-  if (P.p2.family < 0) return launch_family(P.p1, ws, P.carry, st);
+  sat_status s = carry_open(st, ws, ws ? P.carry : 0, P.carry, &c);
+  if (s != SAT_OK) return s;
+  if ((s = carry_epoch(c, st, &P.p1.epoch)) != SAT_OK) return s;
+  if (P.p2.family < 0) return launch_family(P.p1, c.ws, c.bytes, st);
   P.p1.out = tmp;
   P.p2.in = tmp;
-  sat_status s = launch_family(P.p1, ws, P.carry, st);
-  if (s != SAT_OK) return s;
-  return launch_family(P.p2, ws, P.carry, st);
+  if ((s = launch_family(P.p1, c.ws, c.bytes, st)) != SAT_OK) return s;
+  if ((s = carry_epoch(c, st, &P.p2.epoch)) != SAT_OK) return s;
+  return launch_family(P.p2, c.ws, c.bytes, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -524,7 +576,7 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
   return SAT_OK;
 }
 
-int sat_batch_run(void* workspace, int ctas_per_cu, void* stream) {
+int sat_batch_run(void* workspace, void* stream) {
   if (!workspace) return SAT_ERR_NULL;
   BatchPlanInfo info;
   {
@@ -534,14 +586,11 @@ int sat_batch_run(void* workspace, int ctas_per_cu, void* stream) {
     info = f->second;
   }
   if (info.total_tiles == 0) return SAT_OK;
-  if (ctas_per_cu < 0) return SAT_ERR_INVALID;
-  const int bpc = ctas_per_cu > 0 ? ctas_per_cu : batch_blocks_per_cu(info.out_size);
-  int64_t grid = (int64_t)bpc * device_cu_count();
-  if (grid > (int64_t)info.total_tiles) grid = info.total_tiles;
-  uint64_t* carries = (uint64_t*)((char*)workspace + satdev::WS_HEADER_BYTES);
+  hipStream_t st = (hipStream_t)stream;
+  const sat_status s = launch_bump_epoch(workspace, info.carry_words, st);
+  if (s != SAT_OK) return s;
   return launch_batch(info.out_size, (const BatchItem*)((char*)workspace + info.items_off),
-                      (const uint32_t*)((char*)workspace + info.tile_img_off), info.total_tiles, workspace, carries,
-                      info.carry_words, (int)grid, (hipStream_t)stream);
+                      (const uint32_t*)((char*)workspace + info.tile_img_off), info.total_tiles, workspace, st);
 }
 
 int sat_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height,
@@ -583,17 +632,13 @@ int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* gra
   if (planes > 1 && plane_stride < pitch * (int64_t)height) return SAT_ERR_INVALID;
   hipStream_t st = (hipStream_t)stream;
   const size_t need = box_grad_carry_bytes(dtype, width, height, planes);
-  if (!workspace) {
-    const sat_status s = managed_buf(st, WS_CARRY, need, &workspace);
-    if (s != SAT_OK) return s;
-    workspace_bytes = need;
-  } else {
-    if (workspace_bytes < need) return SAT_ERR_WORKSPACE;
-    const sat_status s = launch_clear(workspace, need, st);
-    if (s != SAT_OK) return s;
-  }
-  return launch_box_grad(dtype, boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, workspace,
-                         workspace_bytes, st);
+  CarryCtx c;
+  sat_status s = carry_open(st, workspace, workspace_bytes, need, &c);
+  if (s != SAT_OK) return s;
+  uint32_t ep = 0;
+  if ((s = carry_epoch(c, st, &ep)) != SAT_OK) return s;
+  return launch_box_grad(dtype, boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, c.ws,
+                         c.bytes, ep, st);
 }
 
 }  // extern "C"

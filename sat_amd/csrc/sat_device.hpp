@@ -401,72 +401,32 @@ struct ImgArgs {
 };
 
 // ---------------------------------------------------------------------------
-// Launch-state header at the start of every workspace. A zero-filled
-// workspace is a valid initial state.
+// Workspace header. Carry words follow it. Every launch tags its words with
+// an epoch no earlier launch on the same carry area used:
+//   - calls on a library-managed area take epochs from a host counter in
+//     [1, EPOCH_DEVICE_BASE) and clear the area when the counter wraps;
+//   - launches that must not depend on host state (graph-captured calls and
+//     the batched queue) first run bump_epoch_kernel, which advances the
+//     header's device epoch in [EPOCH_DEVICE_BASE, EPOCH_MAX] and clears the
+//     carry area itself when that range wraps; their workgroups read it;
+//   - calls on a caller-provided area clear it first and count from 1.
 // ---------------------------------------------------------------------------
 struct WsHeader {
-  // Each field the workgroups touch sits on its own 128-byte line: every
-  // workgroup performs one atomic on `counter` and one on `done`, and reads
-  // `epoch`; sharing a line makes the reads queue behind the atomics.
-  uint32_t counter;  // tile tickets handed out in the running launch
+  uint32_t epoch;   // device epoch of the latest bumped launch (0: none yet)
   uint32_t pad0[31];
-  uint32_t done;     // workgroups that finished in the running launch
+  uint32_t fault;   // nonzero once a look-back gave up (sticky until cleared by the host)
   uint32_t pad1[31];
-  uint32_t epoch;    // tag of the running launch (0 is read as 1)
-  uint32_t pad2;
-  uint64_t hwm_words;  // largest carry area (64-bit words) any launch used
-  uint64_t pad3[14];
-  uint32_t fault;      // nonzero once a look-back gave up (sticky until cleared by the host)
-  uint32_t pad4[31];
 };
 constexpr uint32_t EPOCH_MAX = (1u << 30) - 1;
-constexpr int64_t WS_HEADER_BYTES = 512;
+constexpr uint32_t EPOCH_DEVICE_BASE = 1u << 29;
+constexpr int64_t WS_HEADER_BYTES = 256;
 static_assert(sizeof(WsHeader) <= WS_HEADER_BYTES, "header size");
 
-__device__ __forceinline__ uint32_t take_ticket(WsHeader* h) {
-  return __hip_atomic_fetch_add(&h->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t read_epoch(WsHeader* h) {
-  const uint32_t e = __hip_atomic_load(&h->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return e == 0 ? 1u : e;
-}
-
-// Called by every workgroup after its last tile. The workgroup that finishes
-// last resets the ticket counters and advances the epoch; when the epoch would
-// wrap it clears every carry word any launch has used so that no stale tag can
-// be mistaken for a fresh one. Returns through the whole workgroup.
-__device__ __forceinline__ void finish_launch(WsHeader* h, uint64_t* carries, uint64_t words_used,
-                                              uint32_t* s_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // relaxed: the count only elects the workgroup that resets the header; an
-    // acquire/release here would write back and invalidate L2 per workgroup
-    const uint32_t d = __hip_atomic_fetch_add(&h->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t act = 0;
-    if (d == gridDim.x - 1) {
-      const uint32_t e = read_epoch(h);
-      act = (e >= EPOCH_MAX) ? 2u : 1u;
-    }
-    *s_flag = act;
-  }
-  __syncthreads();
-  const uint32_t act = *s_flag;
-  if (act == 0) return;
-  if (act == 2) {
-    uint64_t hwm = __hip_atomic_load(&h->hwm_words, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (words_used > hwm) hwm = words_used;
-    for (uint64_t i = threadIdx.x; i < hwm; i += blockDim.x) st_word(carries + i, 0);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const uint32_t e = read_epoch(h);
-    const uint64_t hwm = __hip_atomic_load(&h->hwm_words, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&h->hwm_words, (act == 2) ? 0 : (words_used > hwm ? words_used : hwm), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&h->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&h->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&h->epoch, act == 2 ? 1u : e + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+// Epoch of the running launch: the kernel argument, or the header's device
+// epoch when the argument is 0.
+__device__ __forceinline__ uint32_t launch_ep4(uint32_t ep_arg, const WsHeader* h) {
+  const uint32_t e = ep_arg ? ep_arg : __hip_atomic_load(&h->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return e << 2;
 }
 
 }  // namespace satdev

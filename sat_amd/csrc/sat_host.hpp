@@ -40,6 +40,7 @@ struct Launch {
   int64_t ldi, ldo;
   int64_t in_plane, out_plane;  // plane strides (elements)
   uint32_t width, height, planes;
+  uint32_t epoch;  // launch epoch; 0 = read the workspace header's device epoch
 };
 
 size_t carry_bytes(const Launch& L);
@@ -53,10 +54,9 @@ struct BatchItem {
 };
 int pair_id(int tin, int tout);  // -1 if unsupported
 sat_status launch_batch(int out_size, const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles,
-                        void* ws, uint64_t* carries, uint64_t words_used, int grid, hipStream_t stream);
+                        void* ws, hipStream_t stream);
 Geometry batch_geometry(size_t out_size);
 int batch_threads(size_t out_size);
-int batch_blocks_per_cu(size_t out_size);
 
 // Box filter (k_box.hip).
 sat_status launch_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width,
@@ -67,11 +67,13 @@ sat_status launch_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, co
                               hipStream_t stream);
 sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x,
                            int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
-                           void* ws, size_t ws_bytes, hipStream_t stream);
+                           void* ws, size_t ws_bytes, uint32_t epoch, hipStream_t stream);
 size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes);
 
 // Zero `bytes` (rounded down to 8) at `p` with a fill kernel (k_sat.hip).
 sat_status launch_clear(void* p, size_t bytes, hipStream_t stream);
+// Advance the header's device epoch (clearing `carry_words` on wrap), k_sat.hip.
+sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream);
 
 // Misc.
 int device_cu_count();

@@ -66,7 +66,7 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   g.vout = aligned_for(L.out, L.ldo, L.out_plane, sizeof(TDst)) ? 1u : 0u;
   g.fault = &((satdev::WsHeader*)ws)->fault;
   hipLaunchKernelGGL(satdev::tiles_kernel<Op>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, g, (uint32_t)tpp,
-                     L.in_plane, L.out_plane, (satdev::WsHeader*)ws, carries, vw + hw);
+                     L.in_plane, L.out_plane, L.epoch, (const satdev::WsHeader*)ws);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 

@@ -330,7 +330,9 @@ struct TileOp {
           q.z = cvt_out<TDst>(y[2]);
           q.w = cvt_out<TDst>(y[3]);
           TDst* o0 = out + (py0 + (REV ? TH - 1 - w * RPT : w * RPT)) * g.ldo + pcol;
-          *(typename Vec4<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo)) = q;
+          // streaming output: nontemporal stores keep it from displacing
+          // the carry words and the input in the caches
+          __builtin_nontemporal_store(q, (typename Vec4<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo)));
         } else {
           store4<TDst>(out + prow * g.ldo, pcol, g.W, prow < (int64_t)g.H, vout, y);
         }
@@ -453,22 +455,20 @@ struct SerialTileOp {
 };
 
 // ---------------------------------------------------------------------------
-// One tile per workgroup; the workgroup count equals the tile count of all
-// planes. Ticket t covers plane t / tpp, tile t % tpp of that plane.
+// One tile per workgroup: tile t = blockIdx.x covers plane t / tpp, tile
+// t % tpp of that plane. Workgroups are dispatched in increasing index order
+// (per XCD), so a tile only ever waits for tiles that are already resident or
+// finished; the bounded look-back turns a violation of that order into a
+// reported fault instead of a hang.
 // ---------------------------------------------------------------------------
 // Four waves per SIMD (at most 128 registers per lane): the tile kernels are
 // latency-bound on their look-backs and need the resident tiles.
 template <class Op>
-__global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(4))) tiles_kernel(ImgArgs g, uint32_t tpp, int64_t in_plane, int64_t out_plane,
-                                                       WsHeader* hdr, uint64_t* carries, uint64_t words_used) {
+__global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(4)))
+    tiles_kernel(ImgArgs g, uint32_t tpp, int64_t in_plane, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr) {
   __shared__ typename Op::Smem sm;
-  __shared__ uint32_t s_ticket, s_ep, s_flag;
-  if (threadIdx.x == 0) {
-    s_ticket = take_ticket(hdr);
-    s_ep = read_epoch(hdr);
-  }
-  __syncthreads();
-  const uint32_t t = s_ticket;
+  const uint32_t ep4 = launch_ep4(ep_arg, hdr);
+  const uint32_t t = blockIdx.x;
   const uint32_t p = t / tpp, lt = t - p * tpp;
   ImgArgs gp = g;
   if (p != 0) {
@@ -477,8 +477,7 @@ __global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(4))
     gp.vcarry = g.vcarry + (int64_t)p * tpp * Op::VWORDS;
     gp.hcarry = g.hcarry + (int64_t)p * tpp * Op::HWORDS;
   }
-  Op::run(gp, lt, s_ep << 2, sm);
-  finish_launch(hdr, carries, words_used, &s_flag);
+  Op::run(gp, lt, ep4, sm);
 }
 
 }  // namespace satdev

@@ -228,9 +228,9 @@ class BatchPlan:
         check(lib().sat_batch_prepare(C.cast(arr, C.c_void_p), len(self.pairs), self.workspace.data_ptr(), nbytes,
                                       torch.cuda.current_stream(dev).cuda_stream), "sat_batch_prepare")
 
-    def run(self, ctas_per_cu: int = 0) -> None:
-        check(lib().sat_batch_run(self.workspace.data_ptr(), ctas_per_cu,
-                                  torch.cuda.current_stream(self.device).cuda_stream), "sat_batch_run")
+    def run(self) -> None:
+        check(lib().sat_batch_run(self.workspace.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream),
+              "sat_batch_run")
 
 
 # ---------------------------------------------------------------------------

@@ -76,8 +76,8 @@ def test_validation_without_device(L):
     assert L.sat_box_sums(5, p, 64, 0, 64, 64, 1, p, 10, p, None) == 3
     assert L.sat_box_sums(6, None, 64, 0, 64, 64, 1, p, 10, p, None) == 1
     assert L.sat_box_grad(6, p, 10, p, p, 32, 0, 64, 64, 1, None, 0, None) == 2
-    assert L.sat_batch_run(None, 0, None) == 1
-    assert L.sat_batch_run(p, 0, None) == 2  # never prepared
+    assert L.sat_batch_run(None, None) == 1
+    assert L.sat_batch_run(p, None) == 2  # never prepared
     # planes with overlapping plane strides
     assert L.sat_integral_planes(3, 0, 6, 6, p, 64, 10, p, 64, 64 * 64, 64, 64, 2, None, 0, None) == 2
 
@@ -102,10 +102,10 @@ def test_batch_validation(L):
 def test_workspace_sizes(L):
     # one-pass f32: 64x256 tiles, 256 column + 1 corner + 64 row carry words each
     tiles = (8192 // 64) * (8192 // 256)
-    assert L.sat_integral_workspace_bytes(3, 6, 6, 8192, 8192) == 512 + tiles * (257 + 64) * 8
+    assert L.sat_integral_workspace_bytes(3, 6, 6, 8192, 8192) == 256 + tiles * (257 + 64) * 8
     # f64: 32x256 tiles, two words per carry
     tiles = (4096 // 32) * (4096 // 256)
-    assert L.sat_integral_workspace_bytes(3, 7, 7, 4096, 4096) == 512 + tiles * (257 + 32) * 16
+    assert L.sat_integral_workspace_bytes(3, 7, 7, 4096, 4096) == 256 + tiles * (257 + 32) * 16
     # two-pass algorithms add a temporary of the output type
     assert L.sat_integral_workspace_bytes(2, 6, 6, 1000, 1000) >= 1024 * 1000 * 4
     assert L.sat_integral_workspace_bytes(0, 0, 4, 1000, 1000) >= 1024 * 1000 * 4

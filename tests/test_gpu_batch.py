@@ -1,0 +1,143 @@
+"""GPU parity of the batched queue (images of uneven sizes, mixed dtype pairs,
+one launch) against the oracle, including hipGraph capture and replay."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def sa():
+    import sat_amd
+
+    sat_amd.lib()
+    return sat_amd
+
+
+def _u32(t):
+    return t.view(torch.int32).cpu().numpy().view(np.uint32)
+
+
+def make_batch(O, dev, n, lo, hi, seed, kinds=("u8", "f32")):
+    rng = np.random.default_rng(seed)
+    imgs = []
+    for i in range(n):
+        h, w = (int(v) for v in rng.integers(lo, hi + 1, 2))
+        kind = kinds[i % len(kinds)]
+        if kind == "u8":
+            x = O.gen("u8", (h, w), seed * 1000 + i)
+            xd = torch.from_numpy(x).to(dev)
+            yd = torch.empty((h, w), dtype=torch.uint32, device=dev)
+        else:
+            x = O.gen("f32", (h, w), seed * 1000 + i)
+            xd = torch.from_numpy(x).to(dev)
+            yd = torch.empty((h, w), dtype=torch.float32, device=dev)
+        imgs.append((kind, x, xd, yd))
+    return imgs
+
+
+def check_batch(O, imgs):
+    for kind, x, _, yd in imgs:
+        if kind == "u8":
+            assert np.array_equal(_u32(yd), O.serial(x, np.uint32)), x.shape
+        else:
+            got = yd.cpu().numpy().astype(np.float64)
+            ref = O.sat_f64(x)
+            assert np.all(np.abs(got - ref) <= 32 * 2.0 ** -24 * ref + 1e-30), x.shape
+
+
+def test_batch_mixed_uneven(O, dev, sa):
+    say("[case] batch 40 images 1..700 mixed u8->u32 / f32")
+    imgs = make_batch(O, dev, 40, 1, 700, 7)
+    plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+    plan.run()
+    check_batch(O, imgs)
+    for _, _, _, yd in imgs:
+        yd.zero_()
+    plan.run()  # a second run of the same plan (new epoch)
+    check_batch(O, imgs)
+    assert sa.lib().sat_workspace_fault(plan.workspace.data_ptr(), None) == 0
+
+
+def test_batch_other_pairs(O, dev, sa):
+    say("[case] batch i8->i32, i32->i32, u32->u32")
+    rng = np.random.default_rng(3)
+    pairs = []
+    exp = []
+    for i in range(12):
+        h, w = (int(v) for v in rng.integers(1, 400, 2))
+        k = i % 3
+        if k == 0:
+            x = O.gen("i8", (h, w), 50 + i)
+            xd, yd = torch.from_numpy(x).to(dev), torch.empty((h, w), dtype=torch.int32, device=dev)
+            exp.append((yd, O.serial(x, np.int32)))
+        elif k == 1:
+            x = O.gen("i32_16", (h, w), 50 + i)
+            xd, yd = torch.from_numpy(x).to(dev), torch.empty((h, w), dtype=torch.int32, device=dev)
+            exp.append((yd, O.serial(x, np.int32)))
+        else:
+            x = O.gen("u32", (h, w), 50 + i)
+            xd = torch.from_numpy(x.view(np.int32)).to(dev).view(torch.uint32)
+            yd = torch.empty((h, w), dtype=torch.uint32, device=dev)
+            exp.append((yd, O.serial(x, np.uint32)))
+        pairs.append((xd, yd))
+    plan = sa.BatchPlan(pairs)
+    plan.run()
+    for yd, e in exp:
+        g = _u32(yd) if yd.dtype == torch.uint32 else yd.cpu().numpy()
+        assert np.array_equal(g, e)
+
+
+def test_batch_f64_outputs(O, dev, sa):
+    say("[case] batch f32->f64 and f64->f64")
+    pairs, refs = [], []
+    for i, (h, w) in enumerate([(33, 700), (300, 41), (129, 257)]):
+        x = O.gen("f64n", (h, w), 70 + i)
+        xd = torch.from_numpy(x).to(dev)
+        yd = torch.empty((h, w), dtype=torch.float64, device=dev)
+        pairs.append((xd, yd))
+        refs.append((yd, x))
+    plan = sa.BatchPlan(pairs)
+    plan.run()
+    for yd, x in refs:
+        got = yd.cpu().numpy()
+        sc = O.serial(np.abs(x), np.float64)
+        assert np.all(np.abs(got - O.sat_f64(x)) <= 32 * 2.0 ** -53 * sc + 1e-300)
+
+
+def test_batch_graph_replay(O, dev, sa):
+    """Capture the run in a hipGraph and replay it several times with new
+    inputs in the same buffers and eager work in between; every replay must
+    match."""
+    say("[case] batch graph capture + replay")
+    imgs = make_batch(O, dev, 12, 20, 600, 11)
+    plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.run()  # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan.run()
+    for rep in range(4):
+        fresh = []
+        for i, (kind, x, xd, yd) in enumerate(imgs):
+            x2 = O.gen("u8" if kind == "u8" else "f32", x.shape, 900 + 37 * rep + i)
+            xd.copy_(torch.from_numpy(x2).to(dev))
+            fresh.append((kind, x2, xd, yd))
+        imgs = fresh
+        _ = torch.ones(1000, device=dev).sum()  # eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        check_batch(O, imgs)
