@@ -141,10 +141,12 @@ def test_managed_path_graph_replay(O, dev, sa):
         sa.integral(x, out=y)
     for rep in range(4):
         xn = O.gen("u8", (H, W), 300 + rep)
-        x.copy_(torch.from_numpy(xn).to(dev))
+        xh = torch.from_numpy(xn).to(dev)
+        torch.cuda.synchronize()
         with torch.cuda.stream(s):
+            x.copy_(xh)
             z = sa.integral(x)  # eager call on the same stream and workspace
-        g.replay()
+            g.replay()
         torch.cuda.synchronize()
         exp = O.serial(xn, np.uint32)
         assert np.array_equal(y.view(torch.int32).cpu().numpy().view(np.uint32), exp)
