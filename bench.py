@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling run: the headline SAT --steps times, then 3 plain copies of the same "
+                         "bytes (the counter calibration), no checks and no CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the SAT path here has no multi-GPU mode "
@@ -112,6 +115,16 @@ def main():
 
     step()
     torch.cuda.synchronize()
+    if args.profile:
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        y2 = torch.empty_like(x)
+        for _ in range(3):
+            y2.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] profile run done: {args.steps} SAT launches, 3 copies of {H * W * 4} bytes")
+        return
     # parity of the measured output (not timed)
     log("[bench] checking the output against the FP64 oracle")
     ref = O.sat_f64(x_np)
@@ -139,6 +152,14 @@ def main():
             dts, _ = r["serial"]
             cpu["serial_1core_GBs"] = algo_bytes / dts / 1e9
 
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # this same workload, scaled by a plain copy of known size (the committed
+    # profile; see its commands)
+    traffic = None
+    tfile = ROOT / "profiles" / "sat_f32_8192_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("traffic_bytes")
+
     out = {
         "metric": "SAT GB/s (in+out bytes) & % HBM peak",
         "value": round(gbs, 2),
@@ -157,7 +178,8 @@ def main():
                    "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
                    "f32_max_rel_err_vs_fp64": rel},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": algo_bytes},
         "cpu_baseline": cpu,
     }
     if extra:
