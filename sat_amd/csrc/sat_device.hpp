@@ -203,12 +203,36 @@ template <typename A> struct Carry {
 // the host reads with sat_workspace_fault().
 constexpr uint32_t SPIN_LIMIT = 1u << 20;
 
+// Wait until a predecessor has published. Only the first calling lane of the
+// wave polls (its own nearest word), so a waiting wave costs one memory
+// request per poll instead of a window of words per lane; false after
+// SPIN_LIMIT polls (recorded in the fault word).
+template <typename A>
+__device__ __forceinline__ bool wait_lead(const uint64_t* base, int64_t e, uint32_t ep4, uint32_t* fault,
+                                          uint32_t& spins) {
+  const int lead = __builtin_ctzll(__ballot(1));
+  for (;;) {
+    uint32_t s = 0;
+    if ((int)(threadIdx.x & 63) == lead) {
+      A v;
+      s = Carry<A>::get(base, e, ep4, v);
+    }
+    if (__builtin_amdgcn_readlane(s, lead) != 0) return true;
+    if (++spins >= SPIN_LIMIT) {
+      __hip_atomic_fetch_or(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 // Decoupled look-back over `n` predecessors, nearest first. `e` is the carry
 // entry of the nearest predecessor, `step` the entry distance to the next one.
 // Each round issues the loads of a window of D predecessors at once (one
 // memory round trip), then folds the longest published prefix of the window:
 // aggregates are summed until an inclusive prefix ends the walk. A window
-// whose nearest word is not yet published is retried after a short sleep.
+// whose nearest word is not yet published is retried once wait_lead has
+// seen it published.
 template <typename A, int D>
 __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t step, uint32_t n, uint32_t ep4,
                                        uint32_t* fault) {
@@ -235,11 +259,7 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
     }
     if (done) break;
     if (used == 0) {
-      if (++spins >= SPIN_LIMIT) {
-        __hip_atomic_fetch_or(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+      if (!wait_lead<A>(base, e, ep4, fault, spins)) break;
       continue;
     }
     e -= (int64_t)used * step;
@@ -300,11 +320,11 @@ __device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t 
       if (done || n[k] == 0) fin[k] = true;
     }
     if (!progress) {
-      if (++spins >= SPIN_LIMIT) {
-        __hip_atomic_fetch_or(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+      int64_t ew = e[0];
+#pragma unroll
+      for (int k = K - 1; k >= 0; --k)
+        if (!fin[k]) ew = e[k];
+      if (!wait_lead<A>(base, ew, ep4, fault, spins)) break;
     }
   }
 }

@@ -44,6 +44,8 @@ struct Launch {
 };
 
 size_t carry_bytes(const Launch& L);
+// XCDs of the current device (workgroup b runs on XCD b % count); 1 if unknown
+int device_xcd_count();
 sat_status launch_family(const Launch& L, void* ws, size_t ws_bytes, hipStream_t stream);
 
 // Batched persistent kernel (k_batch.hip).

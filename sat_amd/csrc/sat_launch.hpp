@@ -65,8 +65,11 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   g.vin = aligned_for(L.in, L.ldi, L.in_plane, sizeof(TSrc)) ? 1u : 0u;
   g.vout = aligned_for(L.out, L.ldo, L.out_plane, sizeof(TDst)) ? 1u : 0u;
   g.fault = &((satdev::WsHeader*)ws)->fault;
+  satdev::TileOrder order;
+  order.nx = (uint32_t)device_xcd_count();
+  order.run = (order.nx > 1 && nTx % order.nx == 0) ? (uint32_t)(nTx / order.nx) : 0u;
   hipLaunchKernelGGL(satdev::tiles_kernel<Op>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, g, (uint32_t)tpp,
-                     L.in_plane, L.out_plane, L.epoch, (const satdev::WsHeader*)ws);
+                     L.in_plane, L.out_plane, L.epoch, (const satdev::WsHeader*)ws, order);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 

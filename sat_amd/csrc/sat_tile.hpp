@@ -32,9 +32,11 @@ namespace satdev {
 enum : int { MODE_ROW = 1, MODE_COL = 2, MODE_SAT = 3 };
 
 // predecessors whose carry words one look-back round reads at once (a lane
-// that follows two column chains reads LB_WINDOW_V per chain)
-constexpr int LB_WINDOW = 8;
-constexpr int LB_WINDOW_V = 4;
+// that follows two column chains reads LB_WINDOW_V per chain). Carry reads
+// cost HBM bandwidth like pixel data, and a wide window mostly reads words
+// past the inclusive one, so the windows are short.
+constexpr int LB_WINDOW = 2;
+constexpr int LB_WINDOW_V = 1;
 
 template <bool C, typename T, typename F> struct cond { using type = T; };
 template <typename T, typename F> struct cond<false, T, F> { using type = F; };
@@ -62,6 +64,8 @@ struct TileOp {
   static constexpr int NVW = !DO_COL ? 1 : (DO_ROW ? NW / 2 : NW);
   static constexpr int CPV = TW / (64 * NVW);  // columns per vertical lane
   static constexpr int HWAVE = NW - 1;
+  // corner look-back: its own wave when there is one to spare
+  static constexpr int CWAVE = (NW >= 3 && NVW <= NW - 2) ? NW - 2 : HWAVE;
   static constexpr int64_t VS = TW + (CORNER ? 1 : 0);  // vertical entries per tile
   static constexpr int64_t VWORDS = DO_COL ? VS * Carry<A>::WORDS : 0;  // per tile
   static constexpr int64_t HWORDS = DO_ROW ? (int64_t)TH * Carry<A>::WORDS : 0;  // per tile
@@ -76,6 +80,7 @@ struct TileOp {
     A rs[DO_ROW ? TH : 1];      // row sums of the tile
     A rc[DO_ROW ? TH : 1];      // row carry of each tile row
     A corner;
+    A ksum;                     // sum of the tile's row carries (corner-slot aggregate)
   };
   struct SmemStage {
     A st[TH * TW];  // XOR-swizzled, see stage_put4 / stage_get
@@ -195,6 +200,7 @@ struct TileOp {
     // ---- row prefix: in-lane over 4 columns, then across the 64 lanes; the
     //      tile's row sums are published as soon as they are known ----
     if constexpr (DO_ROW) {
+      A myrs = zero<A>();  // lane r < RPT: sum of this wave's row r
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         v[r][1] += v[r][0];
@@ -205,11 +211,13 @@ struct TileOp {
         const A rsum = readlane(inc, 63);
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[r][c] += exc;
-        if (lane == r) {
-          const int i = w * RPT + r;
-          Carry<A>::put(g.hcarry, (int64_t)t * TH + i, rsum, ep4 | (J == 0 ? ST_INC : ST_AGG));
-          sm.rs[i] = rsum;
-        }
+        if (lane == r) myrs = rsum;
+      }
+      if (lane < RPT) {
+        // one store instruction per wave publishes its RPT row sums
+        const int i = w * RPT + lane;
+        Carry<A>::put(g.hcarry, (int64_t)t * TH + i, myrs, ep4 | (J == 0 ? ST_INC : ST_AGG));
+        sm.rs[i] = myrs;
       }
       if constexpr (DO_COL) {
         // tile-local column prefix of the row prefixes: the wave-local SAT
@@ -250,30 +258,38 @@ struct TileOp {
           Carry<A>::put(g.hcarry, e, rab + sm.rs[lane], ep4 | ST_INC);
         }
         if constexpr (CORNER) {
-          // row carry = prefix of Rab over the tile's rows; its total feeds
-          // the corner slot of the tiles below
+          // row carry = prefix of Rab over the tile's rows; its total is the
+          // corner-slot aggregate the tiles below need
           const A inc = wave_scan<SCAN>(rab, lane);
           if (lane < TH) sm.rc[lane] = inc;
           const A ksum = readlane(inc, 63);
           if (lane == 0) {
-            const int64_t e = (int64_t)t * VS + TW;
-            A corner = zero<A>();
-            if (I == 0) {
-              Carry<A>::put(g.vcarry, e, ksum, ep4 | ST_INC);
-            } else {
-              Carry<A>::put(g.vcarry, e, ksum, ep4 | ST_AGG);
-              corner = look_back<A, LB_WINDOW>(g.vcarry, e - (int64_t)g.nTx * VS, (int64_t)g.nTx * VS, I, ep4,
-                                                 g.fault);
-              Carry<A>::put(g.vcarry, e, corner + ksum, ep4 | ST_INC);
-            }
-            sm.corner = corner;
+            Carry<A>::put(g.vcarry, (int64_t)t * VS + TW, ksum, ep4 | (I == 0 ? ST_INC : ST_AGG));
+            sm.ksum = ksum;
           }
         } else {
           if (lane < TH) sm.rc[lane] = rab;
         }
       }
+      if constexpr (CORNER) {
+        // the corner look-back only reads the tiles above, so it runs
+        // beside this tile's row look-back
+        if (w == CWAVE && lane == 0) {
+          A corner = zero<A>();
+          if (I > 0) {
+            const int64_t e = (int64_t)t * VS + TW;
+            corner = look_back<A, LB_WINDOW>(g.vcarry, e - (int64_t)g.nTx * VS, (int64_t)g.nTx * VS, I, ep4,
+                                               g.fault);
+          }
+          sm.corner = corner;
+        }
+      }
     }
     __syncthreads();
+    if constexpr (CORNER) {
+      if (I > 0 && tid == 0)
+        Carry<A>::put(g.vcarry, (int64_t)t * VS + TW, sm.corner + sm.ksum, ep4 | ST_INC);
+    }
 
     // ---- per-column and per-row additions ----
     A colAdd[4] = {zero<A>(), zero<A>(), zero<A>(), zero<A>()};
@@ -455,20 +471,39 @@ struct SerialTileOp {
 };
 
 // ---------------------------------------------------------------------------
-// One tile per workgroup: tile t = blockIdx.x covers plane t / tpp, tile
-// t % tpp of that plane. Workgroups are dispatched in increasing index order
-// (per XCD), so a tile only ever waits for tiles that are already resident or
-// finished; the bounded look-back turns a violation of that order into a
-// reported fault instead of a hang.
+// One tile per workgroup. The hardware hands workgroup b to XCD b % nx and each
+// XCD starts its workgroups in increasing b. With order.run > 0 (tile rows
+// split into nx runs of `run` tiles), XCD x takes tile columns
+// [x*run, (x+1)*run) of every tile row, top to bottom: a tile's left and upper
+// neighbours then sit on its own XCD, started just before it, and the row
+// chains cross XCDs only between runs (in XCD order, so the dies settle into
+// a staggered pipeline instead of waiting on each other in a cycle). The map
+// keeps every XCD's sequence increasing in tile index, so a tile still only
+// waits for tiles that are resident or finished; the bounded look-back turns
+// a violation of that order into a reported fault instead of a hang.
+// order.run == 0 keeps tile = blockIdx.x. Tile t covers plane t / tpp, tile
+// t % tpp of that plane (planes stack vertically in the tile grid).
 // ---------------------------------------------------------------------------
+struct TileOrder {
+  uint32_t nx;   // XCDs the workgroups are dealt over
+  uint32_t run;  // tile columns per XCD and tile row (0: identity order)
+};
+
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nTx, TileOrder o) {
+  if (o.run == 0) return b;
+  const uint32_t x = b % o.nx, k = b / o.nx;
+  return (k / o.run) * nTx + x * o.run + (k % o.run);
+}
+
 // Four waves per SIMD (at most 128 registers per lane): the tile kernels are
 // latency-bound on their look-backs and need the resident tiles.
 template <class Op>
 __global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(4)))
-    tiles_kernel(ImgArgs g, uint32_t tpp, int64_t in_plane, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr) {
+    tiles_kernel(ImgArgs g, uint32_t tpp, int64_t in_plane, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr,
+                 TileOrder order) {
   __shared__ typename Op::Smem sm;
   const uint32_t ep4 = launch_ep4(ep_arg, hdr);
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = tile_of_block(blockIdx.x, g.nTx, order);
   const uint32_t p = t / tpp, lt = t - p * tpp;
   ImgArgs gp = g;
   if (p != 0) {
