@@ -186,6 +186,36 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
 int sat_batch_run(void* workspace, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Tile configurations and the autotuner's table. No reference counterpart:
+ * the reference fixes its block shapes through template parameters
+ * (BLOCK_SIZE, SMEM_COUNT, BLOCK_DIM_X/Y, COUNT_PER_THREAD, WARP_COUNT,
+ * e.g. ScanRow-BRLT.cu:24, ScanRowColumn.cu:16,144), which the drop-in
+ * entry points accept but do not use. Here the one-pass SAT (forward and
+ * reverse) and the batch kernel pick a wave64 tile per launch:
+ *   0: 4 waves, 64 x 256 tiles (32 x 256 for 8-byte outputs)
+ *   1: 8 waves, 128 x 256 tiles (64 x 256)
+ *   2: 8 waves, 64 x 256 tiles (32 x 256), half the registers per lane
+ * from a table indexed by output width (4 or 8 bytes) and shape class
+ *   class_x = min(floor(log2(ceil(width / 256))), 8)
+ *   class_y = min(floor(log2(ceil(height * planes / 64))), 11).
+ * The table starts with built-in choices; sat_amd.tune measures every
+ * configuration per class on the device and fills it in. Results of integer
+ * pairs are identical in every configuration. Workspace sizes returned by the
+ * *_workspace_bytes functions fit every configuration.
+ * ------------------------------------------------------------------------- */
+int sat_tile_config_count(void);
+void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class_x, int* class_y);
+/* configuration a class uses now (negative sat_status on bad arguments) */
+int sat_tune_get(int out_bytes, int class_x, int class_y);
+/* cfg = -1 restores the built-in choice */
+int sat_tune_set(int out_bytes, int class_x, int class_y, int cfg);
+/* configuration a launch of this shape would use */
+int sat_tune_pick(int out_bytes, uint32_t width, uint32_t height, uint32_t planes);
+/* batch kernel configuration (used by the next sat_batch_prepare) */
+int sat_batch_tune_get(int out_bytes);
+int sat_batch_tune_set(int out_bytes, int cfg);
+
+/* ---------------------------------------------------------------------------
  * Box-filter readout on summed-area tables (f32 or f64), P planes.
  * boxes: P*nbox records of 4 int32 {y0, x0, y1, x1}, inclusive corners,
  * 0 <= y0 <= y1 < height, 0 <= x0 <= x1 < width.

@@ -62,7 +62,17 @@ SIGNATURES = {
     "sat_box_scatter": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp]),
     "sat_box_grad_workspace_bytes": (_sz, [_i, _u32, _u32, _u32, _u32]),
     "sat_box_grad": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp, _sz, _vp]),
+    "sat_tile_config_count": (_i, []),
+    "sat_tune_class": (None, [_u32, _u32, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sat_tune_get": (_i, [_i, _i, _i]),
+    "sat_tune_set": (_i, [_i, _i, _i, _i]),
+    "sat_tune_pick": (_i, [_i, _u32, _u32, _u32]),
+    "sat_batch_tune_get": (_i, [_i]),
+    "sat_batch_tune_set": (_i, [_i, _i]),
 }
+
+# Tile-configuration table written by sat_amd.tune for this build's target
+TUNED_TABLE = Path(__file__).resolve().parent / "tuned_gfx950.json"
 
 
 class SatError(RuntimeError):
@@ -84,6 +94,10 @@ def lib() -> C.CDLL:
             f.restype = res
             f.argtypes = args
         _lib = L
+        if TUNED_TABLE.exists():
+            from .tune import apply_table
+
+            apply_table(TUNED_TABLE)
     return _lib
 
 

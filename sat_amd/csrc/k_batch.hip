@@ -11,13 +11,17 @@ enum Kind : uint32_t {
   K_I32_I32, K_I32_F64, K_F32_F32, K_F32_F64, K_F64_F64, K_U32_U32, K_COUNT
 };
 
-template <int OUTB> struct BatchCfg;
-template <> struct BatchCfg<4> { static constexpr int NW = 4, RPT = 16; using Ref = float; };
-template <> struct BatchCfg<8> { static constexpr int NW = 4, RPT = 8; using Ref = double; };
+// Tile shape of the batch kernel: tile configuration CFG (as for the one-pass
+// SAT) for OUTB-byte outputs.
+template <int OUTB, int CFG> struct BatchCfg {
+  static constexpr int NW = sathost::TileCfg<CFG>::NW;
+  static constexpr int RPT = (OUTB == 4 ? 16 : 8) / sathost::TileCfg<CFG>::RDIV;
+  using Ref = typename sathost::cond_t<OUTB == 4, float, double>;
+};
 
-template <int OUTB, typename S, typename D>
+template <int OUTB, int CFG, typename S, typename D>
 __device__ __forceinline__ void run_kind(const ImgArgs& g, uint32_t lt, uint32_t ep4, void* smem) {
-  using Cfg = BatchCfg<OUTB>;
+  using Cfg = BatchCfg<OUTB, CFG>;
   using Op = TileOp<S, D, MODE_SAT, false, false, 0, Cfg::NW, Cfg::RPT>;
   Op::run(g, lt, ep4, *reinterpret_cast<typename Op::Smem*>(smem));
 }
@@ -26,11 +30,11 @@ __device__ __forceinline__ void run_kind(const ImgArgs& g, uint32_t lt, uint32_t
 // image tile_img[t]. The launch epoch is the workspace's device epoch,
 // advanced by bump_epoch_kernel before every launch, so the same launch can
 // be replayed from a captured graph.
-template <int OUTB>
-__global__ void __launch_bounds__(64 * BatchCfg<OUTB>::NW) __attribute__((amdgpu_waves_per_eu(4)))
+template <int OUTB, int CFG>
+__global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW)) __attribute__((amdgpu_waves_per_eu(4)))
     batch_kernel(const WsHeader* hdr, const sathost::BatchItem* __restrict__ items,
                  const uint32_t* __restrict__ tile_img) {
-  using Cfg = BatchCfg<OUTB>;
+  using Cfg = BatchCfg<OUTB, CFG>;
   using Ref = typename Cfg::Ref;
   using RefOp = TileOp<Ref, Ref, MODE_SAT, false, false, 0, Cfg::NW, Cfg::RPT>;
   __shared__ typename RefOp::Smem sm;
@@ -43,22 +47,22 @@ __global__ void __launch_bounds__(64 * BatchCfg<OUTB>::NW) __attribute__((amdgpu
   const uint32_t kind = __builtin_amdgcn_readfirstlane(it.kind);
   if constexpr (OUTB == 4) {
     switch (kind) {
-      case K_U8_U32: run_kind<4, uint8_t, uint32_t>(g, lt, ep4, &sm); break;
-      case K_U8_F32: run_kind<4, uint8_t, float>(g, lt, ep4, &sm); break;
-      case K_I8_I32: run_kind<4, int8_t, int32_t>(g, lt, ep4, &sm); break;
-      case K_I8_F32: run_kind<4, int8_t, float>(g, lt, ep4, &sm); break;
-      case K_I32_I32: run_kind<4, int32_t, int32_t>(g, lt, ep4, &sm); break;
-      case K_F32_F32: run_kind<4, float, float>(g, lt, ep4, &sm); break;
-      case K_U32_U32: run_kind<4, uint32_t, uint32_t>(g, lt, ep4, &sm); break;
+      case K_U8_U32: run_kind<4, CFG, uint8_t, uint32_t>(g, lt, ep4, &sm); break;
+      case K_U8_F32: run_kind<4, CFG, uint8_t, float>(g, lt, ep4, &sm); break;
+      case K_I8_I32: run_kind<4, CFG, int8_t, int32_t>(g, lt, ep4, &sm); break;
+      case K_I8_F32: run_kind<4, CFG, int8_t, float>(g, lt, ep4, &sm); break;
+      case K_I32_I32: run_kind<4, CFG, int32_t, int32_t>(g, lt, ep4, &sm); break;
+      case K_F32_F32: run_kind<4, CFG, float, float>(g, lt, ep4, &sm); break;
+      case K_U32_U32: run_kind<4, CFG, uint32_t, uint32_t>(g, lt, ep4, &sm); break;
       default: break;
     }
   } else {
     switch (kind) {
-      case K_U8_F64: run_kind<8, uint8_t, double>(g, lt, ep4, &sm); break;
-      case K_I8_F64: run_kind<8, int8_t, double>(g, lt, ep4, &sm); break;
-      case K_I32_F64: run_kind<8, int32_t, double>(g, lt, ep4, &sm); break;
-      case K_F32_F64: run_kind<8, float, double>(g, lt, ep4, &sm); break;
-      case K_F64_F64: run_kind<8, double, double>(g, lt, ep4, &sm); break;
+      case K_U8_F64: run_kind<8, CFG, uint8_t, double>(g, lt, ep4, &sm); break;
+      case K_I8_F64: run_kind<8, CFG, int8_t, double>(g, lt, ep4, &sm); break;
+      case K_I32_F64: run_kind<8, CFG, int32_t, double>(g, lt, ep4, &sm); break;
+      case K_F32_F64: run_kind<8, CFG, float, double>(g, lt, ep4, &sm); break;
+      case K_F64_F64: run_kind<8, CFG, double, double>(g, lt, ep4, &sm); break;
       default: break;
     }
   }
@@ -85,23 +89,37 @@ int pair_id(int tin, int tout) {
   return -1;
 }
 
-Geometry batch_geometry(size_t out_size) {
-  using namespace satdev;
-  if (out_size == 8) return geom_of<TileOp<double, double, MODE_SAT, false, false, 0, BatchCfg<8>::NW, BatchCfg<8>::RPT>>();
-  return geom_of<TileOp<float, float, MODE_SAT, false, false, 0, BatchCfg<4>::NW, BatchCfg<4>::RPT>>();
+template <int OUTB, int CFG>
+static Geometry batch_geom() {
+  using Ref = typename satdev::BatchCfg<OUTB, CFG>::Ref;
+  return geom_of<satdev::TileOp<Ref, Ref, satdev::MODE_SAT, false, false, 0, satdev::BatchCfg<OUTB, CFG>::NW,
+                                satdev::BatchCfg<OUTB, CFG>::RPT>>();
 }
 
-int batch_threads(size_t out_size) { return 64 * (out_size == 8 ? satdev::BatchCfg<8>::NW : satdev::BatchCfg<4>::NW); }
+Geometry batch_geometry(size_t out_size, int cfg) {
+  if (out_size == 8) return cfg == 1 ? batch_geom<8, 1>() : cfg == 2 ? batch_geom<8, 2>() : batch_geom<8, 0>();
+  return cfg == 1 ? batch_geom<4, 1>() : cfg == 2 ? batch_geom<4, 2>() : batch_geom<4, 0>();
+}
 
-sat_status launch_batch(int out_size, const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles,
-                        void* ws, hipStream_t stream) {
+template <int OUTB, int CFG>
+static void launch_batch_cfg(const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles, void* ws,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL((satdev::batch_kernel<OUTB, CFG>), dim3(total_tiles), dim3(64 * satdev::BatchCfg<OUTB, CFG>::NW),
+                     0, stream, (const satdev::WsHeader*)ws, items, tile_img);
+}
+
+sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* tile_img,
+                        uint32_t total_tiles, void* ws, hipStream_t stream) {
   if (total_tiles == 0) return SAT_OK;
-  if (out_size == 8)
-    hipLaunchKernelGGL(satdev::batch_kernel<8>, dim3(total_tiles), dim3(batch_threads(8)), 0, stream,
-                       (const satdev::WsHeader*)ws, items, tile_img);
-  else
-    hipLaunchKernelGGL(satdev::batch_kernel<4>, dim3(total_tiles), dim3(batch_threads(4)), 0, stream,
-                       (const satdev::WsHeader*)ws, items, tile_img);
+  if (out_size == 8) {
+    if (cfg == 1) launch_batch_cfg<8, 1>(items, tile_img, total_tiles, ws, stream);
+    else if (cfg == 2) launch_batch_cfg<8, 2>(items, tile_img, total_tiles, ws, stream);
+    else launch_batch_cfg<8, 0>(items, tile_img, total_tiles, ws, stream);
+  } else {
+    if (cfg == 1) launch_batch_cfg<4, 1>(items, tile_img, total_tiles, ws, stream);
+    else if (cfg == 2) launch_batch_cfg<4, 2>(items, tile_img, total_tiles, ws, stream);
+    else launch_batch_cfg<4, 0>(items, tile_img, total_tiles, ws, stream);
+  }
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 

@@ -1,4 +1,5 @@
-// One-pass SAT kernels (forward and reverse) and the family geometry table.
+// One-pass SAT kernels (forward and reverse) in tile configuration 0, the
+// configuration dispatch, the fill / epoch kernels and the family geometry table.
 #include "sat_launch.hpp"
 
 namespace satdev {
@@ -45,41 +46,32 @@ sat_status launch_clear(void* p, size_t bytes, hipStream_t stream) {
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 
-template <typename S, typename D, int SC>
-struct FamSat {
-  using Op = TileOp<S, D, MODE_SAT, false, false, SC, kNW, kRPT<D>>;
-  static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
-};
-
-template <typename S, typename D, int SC>
-struct FamSatRev {
-  using Op = TileOp<S, D, MODE_SAT, true, false, SC, kNW, kRPT<D>>;
-  static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
-};
-
 sat_status launch_fam_sat(const Launch& L, void* ws, size_t b, hipStream_t st) {
-  return dispatch_pairs<FamSat>(L, ws, b, st);
-}
-
-// The adjoint is defined for same-type pairs.
-sat_status launch_fam_sat_rev(const Launch& L, void* ws, size_t b, hipStream_t st) {
-  if (L.tin != L.tout) return SAT_ERR_DTYPE;
-  switch (L.tin) {
-    case SAT_F32: return ScanPick<true>::go<FamSatRev, float, float>(L, ws, b, st);
-    case SAT_F64: return ScanPick<true>::go<FamSatRev, double, double>(L, ws, b, st);
-    case SAT_U32: return ScanPick<false>::go<FamSatRev, uint32_t, uint32_t>(L, ws, b, st);
-    case SAT_I32: return ScanPick<false>::go<FamSatRev, int32_t, int32_t>(L, ws, b, st);
-    default: return SAT_ERR_DTYPE;
+  switch (L.cfg) {
+    case 1: return launch_fam_sat_c1(L, ws, b, st);
+    case 2: return launch_fam_sat_c2(L, ws, b, st);
+    default: return launch_sat_cfg<0>(L, ws, b, st);
   }
 }
 
-Geometry geometry(int family, size_t out_size) {
+sat_status launch_fam_sat_rev(const Launch& L, void* ws, size_t b, hipStream_t st) {
+  switch (L.cfg) {
+    case 1: return launch_fam_sat_rev_c1(L, ws, b, st);
+    case 2: return launch_fam_sat_rev_c2(L, ws, b, st);
+    default: return launch_sat_rev_cfg<0>(L, ws, b, st);
+  }
+}
+
+Geometry geometry(int family, size_t out_size, int cfg) {
   const bool b8 = out_size == 8;
   switch (family) {
     case FAM_SAT:
     case FAM_SAT_REV:
-      return b8 ? geom_of<TileOp<double, double, MODE_SAT, false, false, 0, kNW, kRPT<double>>>()
-                : geom_of<TileOp<float, float, MODE_SAT, false, false, 0, kNW, kRPT<float>>>();
+      switch (cfg) {
+        case 1: return sat_geometry_cfg<1>(out_size);
+        case 2: return sat_geometry_cfg<2>(out_size);
+        default: return sat_geometry_cfg<0>(out_size);
+      }
     case FAM_ROWS:
       return b8 ? geom_of<TileOp<double, double, MODE_ROW, false, false, 0, kNW, kRPT<double>>>()
                 : geom_of<TileOp<float, float, MODE_ROW, false, false, 0, kNW, kRPT<float>>>();

@@ -40,9 +40,64 @@ static uint64_t tiles_of(const Geometry& g, uint32_t width, uint32_t height, uin
 }
 
 size_t carry_bytes(const Launch& L) {
-  const Geometry g = geometry(L.family, dtype_size(L.tout));
+  const Geometry g = geometry(L.family, dtype_size(L.tout), L.cfg);
   const uint64_t n = tiles_of(g, L.width, L.height, L.planes);
   return (size_t)(satdev::WS_HEADER_BYTES + n * (uint64_t)(g.vwords + g.hwords) * 8);
+}
+
+static bool tiled_by_cfg(int family) { return family == FAM_SAT || family == FAM_SAT_REV; }
+
+// Workspace bytes that fit the launch in every tile configuration, so that a
+// caller-sized workspace stays valid whatever the tuning table picks.
+static size_t carry_bytes_any_cfg(Launch L) {
+  if (!tiled_by_cfg(L.family)) return carry_bytes(L);
+  size_t m = 0;
+  for (int c = 0; c < TILE_CFGS; ++c) {
+    L.cfg = c;
+    m = std::max(m, carry_bytes(L));
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Tile-configuration table: the autotuner's choice per output width (4 or 8
+// bytes) and shape class; 0 means the built-in choice.
+// ---------------------------------------------------------------------------
+constexpr int TUNE_CX = 9, TUNE_CY = 12;
+static std::atomic<int> g_tune[2][TUNE_CX][TUNE_CY];  // cfg + 1
+static std::atomic<int> g_tune_batch[2];              // cfg + 1
+
+static int floor_log2(uint64_t v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++r;
+  }
+  return r;
+}
+
+static void tune_class(uint32_t width, uint32_t height, uint32_t planes, int* cx, int* cy) {
+  const uint64_t tx = ((uint64_t)width + 255) / 256, ty = ((uint64_t)height * (planes ? planes : 1) + 63) / 64;
+  *cx = std::min(floor_log2(tx ? tx : 1), TUNE_CX - 1);
+  *cy = std::min(floor_log2(ty ? ty : 1), TUNE_CY - 1);
+}
+
+// Built-in choice: narrow images (fewer than 8 tile columns, so no XCD column
+// groups) with at least 4096 rows take 8-wave tiles of twice the height,
+// which halve the vertical carry words per pixel; everything else the 4-wave
+// tile.
+static int builtin_cfg(int cx, int cy) { return (cx < 3 && cy >= 6) ? 1 : 0; }
+
+static int tune_pick(size_t out_size, uint32_t width, uint32_t height, uint32_t planes) {
+  int cx, cy;
+  tune_class(width, height, planes, &cx, &cy);
+  const int v = g_tune[out_size == 8 ? 1 : 0][cx][cy].load(std::memory_order_relaxed);
+  return v > 0 ? v - 1 : builtin_cfg(cx, cy);
+}
+
+static int batch_cfg_pick(size_t out_size) {
+  const int v = g_tune_batch[out_size == 8 ? 1 : 0].load(std::memory_order_relaxed);
+  return v > 0 ? v - 1 : 0;
 }
 
 sat_status launch_family(const Launch& L, void* ws, size_t ws_bytes, hipStream_t st) {
@@ -265,6 +320,7 @@ static sat_status plan_integral(int alg, int scan, int tin, int tout, const void
   P->tmp_plane = 0;
   switch (alg) {
     case SAT_ALG_SINGLE_PASS:
+      a.cfg = tune_pick(osz, width, height, planes);
       break;
     case SAT_ALG_SCANROW_BRLT:
     case SAT_ALG_BRLT_SCANROW: {
@@ -300,7 +356,7 @@ static sat_status plan_integral(int alg, int scan, int tin, int tout, const void
   }
   P->p1 = a;
   P->p2 = b;
-  size_t carry = carry_bytes(a);
+  size_t carry = carry_bytes_any_cfg(a);
   if (b.family >= 0) carry = std::max(carry, carry_bytes(b));
   P->carry = carry;
   P->tmp_off = (size_t)round_up((int64_t)carry, 256);
@@ -338,6 +394,7 @@ static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_
 // ---------------------------------------------------------------------------
 struct BatchPlanInfo {
   int out_size = 0;
+  int cfg = 0;
   uint32_t total_tiles = 0;
   uint64_t carry_words = 0;
   size_t items_off = 0, tile_img_off = 0, total = 0;
@@ -345,8 +402,8 @@ struct BatchPlanInfo {
 static std::mutex g_batch_mu;
 static std::map<uintptr_t, BatchPlanInfo> g_batch;
 
-static sat_status batch_layout(const sat_image* im, int count, BatchPlanInfo* info, std::vector<BatchItem>* items,
-                               std::vector<uint32_t>* tile_img) {
+static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPlanInfo* info,
+                               std::vector<BatchItem>* items, std::vector<uint32_t>* tile_img) {
   if (count < 0) return SAT_ERR_INVALID;
   if (count > 0 && !im) return SAT_ERR_NULL;
   int osz = 0;
@@ -360,11 +417,13 @@ static sat_status batch_layout(const sat_image* im, int count, BatchPlanInfo* in
     if (c != SAT_OK) return c;
   }
   if (osz == 0) osz = 4;
-  const Geometry g = batch_geometry(osz);
+  if (cfg < 0) cfg = batch_cfg_pick(osz);
+  const Geometry g = batch_geometry(osz, cfg);
   uint64_t tiles = 0;
   for (int i = 0; i < count; ++i) tiles += tiles_of(g, im[i].width, im[i].height, 1);
   if (tiles > 0x7fffffffull) return SAT_ERR_INVALID;
   info->out_size = osz;
+  info->cfg = cfg;
   info->total_tiles = (uint32_t)tiles;
   info->carry_words = tiles * (uint64_t)(g.vwords + g.hwords);
   info->items_off = (size_t)round_up(satdev::WS_HEADER_BYTES + (int64_t)info->carry_words * 8, 256);
@@ -428,6 +487,46 @@ const char* sat_status_string(int status) {
     case SAT_ERR_UNSUPPORTED: return "not supported by this build";
     default: return "unknown status";
   }
+}
+
+int sat_tile_config_count(void) { return TILE_CFGS; }
+
+void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class_x, int* class_y) {
+  int cx, cy;
+  tune_class(width, height, planes, &cx, &cy);
+  if (class_x) *class_x = cx;
+  if (class_y) *class_y = cy;
+}
+
+int sat_tune_get(int out_bytes, int class_x, int class_y) {
+  if ((out_bytes != 4 && out_bytes != 8) || class_x < 0 || class_x >= TUNE_CX || class_y < 0 || class_y >= TUNE_CY)
+    return -SAT_ERR_INVALID;
+  const int v = g_tune[out_bytes == 8 ? 1 : 0][class_x][class_y].load(std::memory_order_relaxed);
+  return v > 0 ? v - 1 : builtin_cfg(class_x, class_y);
+}
+
+int sat_tune_set(int out_bytes, int class_x, int class_y, int cfg) {
+  if ((out_bytes != 4 && out_bytes != 8) || class_x < 0 || class_x >= TUNE_CX || class_y < 0 || class_y >= TUNE_CY ||
+      cfg < -1 || cfg >= TILE_CFGS)
+    return SAT_ERR_INVALID;
+  g_tune[out_bytes == 8 ? 1 : 0][class_x][class_y].store(cfg + 1, std::memory_order_relaxed);
+  return SAT_OK;
+}
+
+int sat_tune_pick(int out_bytes, uint32_t width, uint32_t height, uint32_t planes) {
+  if (out_bytes != 4 && out_bytes != 8) return -SAT_ERR_INVALID;
+  return tune_pick((size_t)out_bytes, width, height, planes);
+}
+
+int sat_batch_tune_get(int out_bytes) {
+  if (out_bytes != 4 && out_bytes != 8) return -SAT_ERR_INVALID;
+  return batch_cfg_pick((size_t)out_bytes);
+}
+
+int sat_batch_tune_set(int out_bytes, int cfg) {
+  if ((out_bytes != 4 && out_bytes != 8) || cfg < -1 || cfg >= TILE_CFGS) return SAT_ERR_INVALID;
+  g_tune_batch[out_bytes == 8 ? 1 : 0].store(cfg + 1, std::memory_order_relaxed);
+  return SAT_OK;
 }
 
 int sat_workspace_fault(void* workspace, void* stream) {
@@ -523,7 +622,7 @@ int sat_integral(int alg, int scan, int tin, int tout, const void* in, int64_t i
 size_t sat_integral_reverse_planes_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes) {
   Launch L = make_launch(FAM_SAT_REV, 0, dtype, dtype, nullptr, width, nullptr, width, width, height);
   L.planes = planes;
-  return carry_bytes(L);
+  return carry_bytes_any_cfg(L);
 }
 
 size_t sat_integral_reverse_workspace_bytes(int dtype, uint32_t width, uint32_t height) {
@@ -545,6 +644,7 @@ int sat_integral_reverse_planes(int scan, int dtype, const void* in, int64_t in_
   L.in_plane = in_plane_stride;
   L.out_plane = out_plane_stride;
   L.planes = planes;
+  L.cfg = tune_pick(dtype_size(dtype), width, height, planes);
   return run_with_ws(L, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
@@ -555,9 +655,15 @@ int sat_integral_reverse(int scan, int dtype, const void* in, int64_t in_pitch, 
 }
 
 size_t sat_batch_workspace_bytes(const sat_image* images, int count) {
-  BatchPlanInfo info;
-  if (batch_layout(images, count, &info, nullptr, nullptr) != SAT_OK) return 0;
-  return info.total;
+  // the largest layout over the tile configurations: valid whatever the
+  // tuning table picks at sat_batch_prepare
+  size_t m = 0;
+  for (int c = 0; c < TILE_CFGS; ++c) {
+    BatchPlanInfo info;
+    if (batch_layout(images, count, c, &info, nullptr, nullptr) != SAT_OK) return 0;
+    m = std::max(m, info.total);
+  }
+  return m;
 }
 
 int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_t workspace_bytes, void* stream) {
@@ -565,10 +671,10 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
   BatchPlanInfo info;
   std::vector<BatchItem> items;
   std::vector<uint32_t> tile_img;
-  sat_status s = batch_layout(images, count, &info, &items, &tile_img);
+  sat_status s = batch_layout(images, count, -1, &info, &items, &tile_img);
   if (s != SAT_OK) return s;
   if (workspace_bytes < info.total) return SAT_ERR_WORKSPACE;
-  const Geometry g = batch_geometry(info.out_size);
+  const Geometry g = batch_geometry(info.out_size, info.cfg);
   uint64_t* carries = (uint64_t*)((char*)workspace + satdev::WS_HEADER_BYTES);
   for (auto& it : items) {
     it.g.fault = &((satdev::WsHeader*)workspace)->fault;
@@ -604,7 +710,7 @@ int sat_batch_run(void* workspace, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   const sat_status s = launch_bump_epoch(workspace, info.carry_words, st);
   if (s != SAT_OK) return s;
-  return launch_batch(info.out_size, (const BatchItem*)((char*)workspace + info.items_off),
+  return launch_batch(info.out_size, info.cfg, (const BatchItem*)((char*)workspace + info.items_off),
                       (const uint32_t*)((char*)workspace + info.tile_img_off), info.total_tiles, workspace, st);
 }
 

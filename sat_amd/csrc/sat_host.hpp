@@ -28,7 +28,10 @@ struct Geometry {
   int64_t vwords, hwords;  // carry words per tile
   int threads;
 };
-Geometry geometry(int family, size_t out_size);
+// Tile configurations of the one-pass families (FAM_SAT, FAM_SAT_REV) and of
+// the batch kernel; other families have one shape. cfg in [0, TILE_CFGS).
+constexpr int TILE_CFGS = 3;
+Geometry geometry(int family, size_t out_size, int cfg = 0);
 
 // One launch of a family over `planes` images of identical shape.
 struct Launch {
@@ -41,6 +44,7 @@ struct Launch {
   int64_t in_plane, out_plane;  // plane strides (elements)
   uint32_t width, height, planes;
   uint32_t epoch;  // launch epoch; 0 = read the workspace header's device epoch
+  int cfg = 0;     // tile configuration (one-pass families)
 };
 
 size_t carry_bytes(const Launch& L);
@@ -55,10 +59,9 @@ struct BatchItem {
   uint32_t tile_base;  // first global tile of this image
 };
 int pair_id(int tin, int tout);  // -1 if unsupported
-sat_status launch_batch(int out_size, const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles,
-                        void* ws, hipStream_t stream);
-Geometry batch_geometry(size_t out_size);
-int batch_threads(size_t out_size);
+sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* tile_img,
+                        uint32_t total_tiles, void* ws, hipStream_t stream);
+Geometry batch_geometry(size_t out_size, int cfg);
 
 // Box filter (k_box.hip).
 sat_status launch_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width,

@@ -14,6 +14,20 @@ template <typename TDst> constexpr int kRPT = sizeof(arith_t<TDst>) == 4 ? 16 : 
 template <typename TDst> constexpr int kSerialCW = sizeof(arith_t<TDst>) == 4 ? 64 : 32;
 constexpr int kNW = 4;
 
+// Tile configurations of the one-pass SAT (chosen per launch by the tuning
+// table):
+//   0: 4 waves x kRPT rows  (64 x 256 tiles for 4-byte outputs, 32 x 256 for 8-byte)
+//   1: 8 waves x kRPT rows  (128 x 256 / 64 x 256): half the carry words per pixel
+//      in the vertical direction, for tall images
+//   2: 8 waves x kRPT/2 rows (64 x 256 / 32 x 256): half the accumulators per lane
+template <bool C, typename T, typename F> using cond_t = typename satdev::cond<C, T, F>::type;
+
+template <int CFG> struct TileCfg;
+template <> struct TileCfg<0> { static constexpr int NW = 4, RDIV = 1; };
+template <> struct TileCfg<1> { static constexpr int NW = 8, RDIV = 1; };
+template <> struct TileCfg<2> { static constexpr int NW = 8, RDIV = 2; };
+template <int CFG, typename TDst> constexpr int cfg_rpt = kRPT<TDst> / TileCfg<CFG>::RDIV;
+
 template <class Op>
 inline Geometry geom_of() {
   Geometry g;
@@ -114,5 +128,48 @@ inline sat_status dispatch_pairs(const Launch& L, void* ws, size_t b, hipStream_
 #undef SAT_X
   return SAT_ERR_DTYPE;
 }
+
+// One-pass SAT (forward and reverse) in tile configuration CFG; every
+// configuration lives in its own translation unit (k_sat*.hip).
+template <int CFG> struct FamSatCfg {
+  template <typename S, typename D, int SC> struct Fwd {
+    using Op = satdev::TileOp<S, D, satdev::MODE_SAT, false, false, SC, TileCfg<CFG>::NW, cfg_rpt<CFG, D>>;
+    static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
+  };
+  template <typename S, typename D, int SC> struct Rev {
+    using Op = satdev::TileOp<S, D, satdev::MODE_SAT, true, false, SC, TileCfg<CFG>::NW, cfg_rpt<CFG, D>>;
+    static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
+  };
+};
+
+template <int CFG>
+inline sat_status launch_sat_cfg(const Launch& L, void* ws, size_t b, hipStream_t st) {
+  return dispatch_pairs<FamSatCfg<CFG>::template Fwd>(L, ws, b, st);
+}
+
+// The adjoint is defined for same-type pairs.
+template <int CFG>
+inline sat_status launch_sat_rev_cfg(const Launch& L, void* ws, size_t b, hipStream_t st) {
+  if (L.tin != L.tout) return SAT_ERR_DTYPE;
+  using F = FamSatCfg<CFG>;
+  switch (L.tin) {
+    case SAT_F32: return ScanPick<true>::go<F::template Rev, float, float>(L, ws, b, st);
+    case SAT_F64: return ScanPick<true>::go<F::template Rev, double, double>(L, ws, b, st);
+    case SAT_U32: return ScanPick<false>::go<F::template Rev, uint32_t, uint32_t>(L, ws, b, st);
+    case SAT_I32: return ScanPick<false>::go<F::template Rev, int32_t, int32_t>(L, ws, b, st);
+    default: return SAT_ERR_DTYPE;
+  }
+}
+
+template <int CFG>
+inline Geometry sat_geometry_cfg(size_t out_size) {
+  if (out_size == 8) return geom_of<typename FamSatCfg<CFG>::template Fwd<double, double, 0>::Op>();
+  return geom_of<typename FamSatCfg<CFG>::template Fwd<float, float, 0>::Op>();
+}
+
+sat_status launch_fam_sat_c1(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev_c1(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_c2(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev_c2(const Launch& L, void* ws, size_t b, hipStream_t st);
 
 }  // namespace sathost

@@ -70,7 +70,10 @@ struct TileOp {
   static constexpr int64_t VWORDS = DO_COL ? VS * Carry<A>::WORDS : 0;  // per tile
   static constexpr int64_t HWORDS = DO_ROW ? (int64_t)TH * Carry<A>::WORDS : 0;  // per tile
   static_assert(!(TOUT && (REV || DO_COL)), "transposed store is a row-scan pass only");
-  static_assert(TH <= 64, "one lane per tile row in the horizontal look-back");
+  // the horizontal look-back wave follows KH rows per lane: rows l, l+64, ...
+  static constexpr int KH = (TH + 63) / 64;
+  static_assert(TH <= 64 || TH % 64 == 0, "tile rows per horizontal lane");
+  static_assert(!TOUT || TH <= 64, "one lane per tile row in the transposed store");
   static_assert(!CORNER || NW >= 2, "the 2-D pass splits look-backs over waves");
   static_assert(TW % (64 * NVW) == 0, "vertical lanes");
 
@@ -251,24 +254,44 @@ struct TileOp {
     }
     if constexpr (DO_ROW) {
       if (w == HWAVE) {
-        A rab = zero<A>();
-        if (lane < TH && J > 0) {
-          const int64_t e = (int64_t)t * TH + lane;
-          rab = look_back<A, LB_WINDOW>(g.hcarry, e - TH, TH, J, ep4, g.fault);
-          Carry<A>::put(g.hcarry, e, rab + sm.rs[lane], ep4 | ST_INC);
+        A rab[KH];
+#pragma unroll
+        for (int k = 0; k < KH; ++k) rab[k] = zero<A>();
+        if (J > 0) {
+          if constexpr (KH == 1) {
+            if (lane < TH) {
+              const int64_t e = (int64_t)t * TH + lane;
+              rab[0] = look_back<A, LB_WINDOW>(g.hcarry, e - TH, TH, J, ep4, g.fault);
+              Carry<A>::put(g.hcarry, e, rab[0] + sm.rs[lane], ep4 | ST_INC);
+            }
+          } else {
+            int64_t e[KH];
+#pragma unroll
+            for (int k = 0; k < KH; ++k) e[k] = (int64_t)(t - 1) * TH + 64 * k + lane;
+            look_back_k<A, LB_WINDOW, KH>(g.hcarry, e, TH, J, ep4, g.fault, rab);
+#pragma unroll
+            for (int k = 0; k < KH; ++k)
+              Carry<A>::put(g.hcarry, (int64_t)t * TH + 64 * k + lane, rab[k] + sm.rs[64 * k + lane], ep4 | ST_INC);
+          }
         }
         if constexpr (CORNER) {
           // row carry = prefix of Rab over the tile's rows; its total is the
           // corner-slot aggregate the tiles below need
-          const A inc = wave_scan<SCAN>(rab, lane);
-          if (lane < TH) sm.rc[lane] = inc;
-          const A ksum = readlane(inc, 63);
+          A base = zero<A>();
+#pragma unroll
+          for (int k = 0; k < KH; ++k) {
+            const A inc = wave_scan<SCAN>(rab[k], lane);
+            if (64 * k + lane < TH) sm.rc[64 * k + lane] = base + inc;
+            base += readlane(inc, 63);
+          }
           if (lane == 0) {
-            Carry<A>::put(g.vcarry, (int64_t)t * VS + TW, ksum, ep4 | (I == 0 ? ST_INC : ST_AGG));
-            sm.ksum = ksum;
+            Carry<A>::put(g.vcarry, (int64_t)t * VS + TW, base, ep4 | (I == 0 ? ST_INC : ST_AGG));
+            sm.ksum = base;
           }
         } else {
-          if (lane < TH) sm.rc[lane] = rab;
+#pragma unroll
+          for (int k = 0; k < KH; ++k)
+            if (64 * k + lane < TH) sm.rc[64 * k + lane] = rab[k];
         }
       }
       if constexpr (CORNER) {

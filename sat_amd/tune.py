@@ -1,0 +1,212 @@
+"""Autotuner for the tile configuration of the one-pass SAT and the batch kernel.
+
+The library picks one of ``sat_tile_config_count()`` wave64 tile
+configurations per launch from a table indexed by output width (4 or 8 bytes)
+and shape class (``sat_tune_class``). This module measures every
+configuration on the current device for representative shapes of each class,
+checks that the configurations agree (integer pairs bit-for-bit, floating
+point within the summation-order tolerance), and writes the fastest choice per
+class as JSON; ``apply_table`` loads such a file into the library (the package
+does so at load time for ``tuned_gfx950.json``).
+
+    python -m sat_amd.tune                 # full table -> sat_amd/tuned_gfx950.json
+    python -m sat_amd.tune --quick --out /tmp/t.json
+
+The reference has no tuner: its block shapes are template parameters fixed by
+the caller (e.g. ScanRow-BRLT.cu:24, ScanRowColumn.cu:16,144).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import sys
+from pathlib import Path
+
+CFG_NAMES = {0: "4 waves, 64x256 (32x256 for 8-byte outputs)",
+             1: "8 waves, 128x256 (64x256)",
+             2: "8 waves, 64x256 (32x256), half the registers per lane"}
+
+# (height, width, planes) shapes that stand for their class; the BASELINE
+# workloads plus square, tall, wide and plane-stacked sweeps
+FULL_SHAPES = [
+    (2048, 2048, 1), (4096, 4096, 1), (8192, 8192, 1), (16384, 16384, 1),
+    (1024, 1024, 1), (512, 512, 1), (256, 256, 1),
+    (65536, 256, 1), (16384, 256, 1), (65536, 1024, 1), (16384, 1024, 1), (8192, 512, 1),
+    (256, 65536, 1), (1024, 16384, 1), (2048, 32768, 1),
+    (1024, 1024, 96), (512, 512, 64), (64, 64, 1024),
+]
+QUICK_SHAPES = [(8192, 8192, 1), (65536, 256, 1), (1024, 1024, 96)]
+
+
+def classes_of(L, width: int, height: int, planes: int):
+    cx, cy = C.c_int(), C.c_int()
+    L.sat_tune_class(width, height, planes, C.byref(cx), C.byref(cy))
+    return cx.value, cy.value
+
+
+def apply_table(path) -> int:
+    """Load a tuned table (JSON written by :func:`tune`) into the library;
+    returns the number of entries applied."""
+    from . import lib
+
+    L = lib()
+    doc = json.loads(Path(path).read_text())
+    n = 0
+    for e in doc.get("entries", []):
+        if L.sat_tune_set(int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]), int(e["cfg"])) == 0:
+            n += 1
+    for e in doc.get("batch", []):
+        if L.sat_batch_tune_set(int(e["out_bytes"]), int(e["cfg"])) == 0:
+            n += 1
+    return n
+
+
+def _time(fn, reps: int) -> float:
+    """Median microseconds of `reps` back-to-back groups of 5 calls."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(5):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e) * 1000 / 5)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def _tune_shape(L, sa, H, W, P, out_bytes, reps, log):
+    import torch
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(H * 131 + W * 7 + P)
+    cx, cy = classes_of(L, W, H, P)
+    if out_bytes == 4:
+        # timing on f32, invariance on u8 -> u32 (bit-exact in every configuration)
+        x = torch.rand((P, H, W), generator=g).to(dev)
+        xi = torch.randint(0, 256, (P, H, W), generator=g, dtype=torch.uint8).to(dev)
+        out_dt, int_dt = torch.float32, torch.uint32
+    else:
+        x = torch.randn((P, H, W), generator=g, dtype=torch.float64).to(dev)
+        xi = torch.randint(-(1 << 20), 1 << 20, (P, H, W), generator=g, dtype=torch.int32).to(dev)
+        out_dt, int_dt = torch.float64, torch.float64  # i32 -> f64 is exact at these magnitudes
+    y = torch.empty(x.shape, dtype=out_dt, device=dev)
+    yi = torch.empty(x.shape, dtype=int_dt, device=dev)
+    times, ref, ref_f = [], None, None
+    for cfg in range(L.sat_tile_config_count()):
+        sa.check(L.sat_tune_set(out_bytes, cx, cy, cfg), "sat_tune_set")
+        assert L.sat_tune_pick(out_bytes, W, H, P) == cfg
+        sa.integral(xi, out_dtype=int_dt, out=yi)
+        sa.integral(x, out=y)
+        torch.cuda.synchronize()
+        got = yi.view(torch.int32) if int_dt == torch.uint32 else yi
+        if ref is None:
+            ref, ref_f = got.clone(), y.clone()
+        else:
+            if not torch.equal(got, ref):
+                raise AssertionError(f"tile configuration {cfg} changes integer results at {P}x{H}x{W}")
+            scale = torch.cumsum(torch.cumsum(x.abs().double(), -1), -2)
+            u = 2.0 ** -24 if out_bytes == 4 else 2.0 ** -53
+            err = float(((y.double() - ref_f.double()).abs() / scale.clamp_min(1e-300)).max())
+            if err > 64 * u:
+                raise AssertionError(f"tile configuration {cfg} differs by {err:.2e} (relative) at {P}x{H}x{W}")
+        times.append(_time(lambda: sa.integral(x, out=y), reps))
+    best = min(range(len(times)), key=lambda c: times[c])
+    log(f"[tune] {P}x{H}x{W} out {out_bytes}B class ({cx},{cy}): "
+        + "  ".join(f"cfg{c} {t:8.1f} us" for c, t in enumerate(times)) + f"  -> cfg{best}")
+    return {"out_bytes": out_bytes, "class_x": cx, "class_y": cy, "cfg": best, "shape": [H, W], "planes": P,
+            "us": [round(t, 2) for t in times]}
+
+
+def _tune_batch(L, sa, out_bytes, n, reps, log):
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(20260400 + out_bytes)
+    pairs = []
+    for i in range(n):
+        h, w = (int(v) for v in rng.integers(64, 2049, 2))
+        if out_bytes == 4:
+            xd = torch.rand((h, w), device=dev) if i % 2 else torch.randint(0, 256, (h, w), dtype=torch.uint8,
+                                                                           device=dev)
+            yd = torch.empty((h, w), dtype=torch.float32 if i % 2 else torch.uint32, device=dev)
+        else:
+            xd = torch.randn((h, w), dtype=torch.float64, device=dev)
+            yd = torch.empty((h, w), dtype=torch.float64, device=dev)
+        pairs.append((xd, yd))
+    times = []
+    outs = None
+    for cfg in range(L.sat_tile_config_count()):
+        sa.check(L.sat_batch_tune_set(out_bytes, cfg), "sat_batch_tune_set")
+        plan = sa.BatchPlan(pairs)
+        plan.run()
+        torch.cuda.synchronize()
+        ints = [yd.clone() for (xd, yd) in pairs if yd.dtype == torch.uint32]
+        if outs is None:
+            outs = ints
+        else:
+            for a, b in zip(outs, ints):
+                if not torch.equal(a.view(torch.int32), b.view(torch.int32)):
+                    raise AssertionError(f"batch tile configuration {cfg} changes integer results")
+        times.append(_time(plan.run, reps))
+    best = min(range(len(times)), key=lambda c: times[c])
+    log(f"[tune] batch of {n} out {out_bytes}B: " + "  ".join(f"cfg{c} {t:8.1f} us" for c, t in enumerate(times))
+        + f"  -> cfg{best}")
+    return {"out_bytes": out_bytes, "cfg": best, "images": n, "us": [round(t, 2) for t in times]}
+
+
+def tune(shapes=None, out_bytes=(4, 8), batch_images=256, reps=7, log=print) -> dict:
+    """Measure every tile configuration per shape class on cuda:0 and return the
+    table (classes not covered keep the built-in choice)."""
+    import torch
+
+    import sat_amd as sa
+
+    L = sa.lib()
+    shapes = FULL_SHAPES if shapes is None else shapes
+    entries = {}
+    try:
+        for ob in out_bytes:
+            for (H, W, P) in shapes:
+                if ob == 8 and H * W * P > (1 << 28):
+                    continue  # 8-byte outputs: keep each sweep image under 2 GiB
+                e = _tune_shape(L, sa, H, W, P, ob, reps, log)
+                entries[(ob, e["class_x"], e["class_y"])] = e  # the last shape of a class decides
+        batch = [_tune_batch(L, sa, ob, batch_images, reps, log) for ob in out_bytes] if batch_images else []
+    finally:
+        for ob in (4, 8):
+            L.sat_batch_tune_set(ob, -1)
+            for cx in range(9):
+                for cy in range(12):
+                    L.sat_tune_set(ob, cx, cy, -1)
+    torch.cuda.synchronize()
+    return {"arch": torch.cuda.get_device_properties(0).gcnArchName.split(":")[0],
+            "generated_by": "python -m sat_amd.tune", "configs": CFG_NAMES,
+            "entries": sorted(entries.values(), key=lambda e: (e["out_bytes"], e["class_x"], e["class_y"])),
+            "batch": batch}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=str(Path(__file__).resolve().parent / "tuned_gfx950.json"))
+    ap.add_argument("--quick", action="store_true", help="three shapes, 4-byte outputs, small batch")
+    ap.add_argument("--reps", type=int, default=7)
+    a = ap.parse_args(argv)
+    log = lambda *m: print(*m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.quick:
+        doc = tune(QUICK_SHAPES, out_bytes=(4,), batch_images=64, reps=a.reps, log=log)
+    else:
+        doc = tune(reps=a.reps, log=log)
+    Path(a.out).write_text(json.dumps(doc, indent=1) + "\n")
+    log(f"[tune] wrote {a.out}: {len(doc['entries'])} classes, {len(doc['batch'])} batch entries")
+
+
+if __name__ == "__main__":
+    main()
