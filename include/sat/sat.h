@@ -197,18 +197,19 @@ int sat_batch_run(void* workspace, void* stream);
  *   2: 8 waves, 64 x 256 tiles (32 x 256), half the registers per lane
  * from a table indexed by output width (4 or 8 bytes) and shape class
  *   class_x = min(floor(log2(ceil(width / 256))), 8)
- *   class_y = min(floor(log2(ceil(height * planes / 64))), 11).
+ *   class_y = min(floor(log2(ceil(height / 64))), 11)
+ *   class_p = 0, 1, 2, 3 for 1, 2-3, 4-15, 16+ planes.
  * The table starts with built-in choices; sat_amd.tune measures every
  * configuration per class on the device and fills it in. Results of integer
  * pairs are identical in every configuration. Workspace sizes returned by the
  * *_workspace_bytes functions fit every configuration.
  * ------------------------------------------------------------------------- */
 int sat_tile_config_count(void);
-void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class_x, int* class_y);
+void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class_x, int* class_y, int* class_p);
 /* configuration a class uses now (negative sat_status on bad arguments) */
-int sat_tune_get(int out_bytes, int class_x, int class_y);
+int sat_tune_get(int out_bytes, int class_x, int class_y, int class_p);
 /* cfg = -1 restores the built-in choice */
-int sat_tune_set(int out_bytes, int class_x, int class_y, int cfg);
+int sat_tune_set(int out_bytes, int class_x, int class_y, int class_p, int cfg);
 /* configuration a launch of this shape would use */
 int sat_tune_pick(int out_bytes, uint32_t width, uint32_t height, uint32_t planes);
 /* batch kernel configuration (used by the next sat_batch_prepare) */

@@ -63,9 +63,9 @@ SIGNATURES = {
     "sat_box_grad_workspace_bytes": (_sz, [_i, _u32, _u32, _u32, _u32]),
     "sat_box_grad": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp, _sz, _vp]),
     "sat_tile_config_count": (_i, []),
-    "sat_tune_class": (None, [_u32, _u32, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
-    "sat_tune_get": (_i, [_i, _i, _i]),
-    "sat_tune_set": (_i, [_i, _i, _i, _i]),
+    "sat_tune_class": (None, [_u32, _u32, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sat_tune_get": (_i, [_i, _i, _i, _i]),
+    "sat_tune_set": (_i, [_i, _i, _i, _i, _i]),
     "sat_tune_pick": (_i, [_i, _u32, _u32, _u32]),
     "sat_batch_tune_get": (_i, [_i]),
     "sat_batch_tune_set": (_i, [_i, _i]),

@@ -63,9 +63,9 @@ static size_t carry_bytes_any_cfg(Launch L) {
 // Tile-configuration table: the autotuner's choice per output width (4 or 8
 // bytes) and shape class; 0 means the built-in choice.
 // ---------------------------------------------------------------------------
-constexpr int TUNE_CX = 9, TUNE_CY = 12;
-static std::atomic<int> g_tune[2][TUNE_CX][TUNE_CY];  // cfg + 1
-static std::atomic<int> g_tune_batch[2];              // cfg + 1
+constexpr int TUNE_CX = 9, TUNE_CY = 12, TUNE_CP = 4;
+static std::atomic<int> g_tune[2][TUNE_CX][TUNE_CY][TUNE_CP];  // cfg + 1
+static std::atomic<int> g_tune_batch[2];                        // cfg + 1
 
 static int floor_log2(uint64_t v) {
   int r = 0;
@@ -76,23 +76,38 @@ static int floor_log2(uint64_t v) {
   return r;
 }
 
-static void tune_class(uint32_t width, uint32_t height, uint32_t planes, int* cx, int* cy) {
-  const uint64_t tx = ((uint64_t)width + 255) / 256, ty = ((uint64_t)height * (planes ? planes : 1) + 63) / 64;
+// class_x: tile columns (log2), class_y: tile rows of one plane (log2),
+// class_p: planes 1 / 2-3 / 4-15 / 16+
+static void tune_class(uint32_t width, uint32_t height, uint32_t planes, int* cx, int* cy, int* cp) {
+  const uint64_t tx = ((uint64_t)width + 255) / 256, ty = ((uint64_t)height + 63) / 64;
   *cx = std::min(floor_log2(tx ? tx : 1), TUNE_CX - 1);
   *cy = std::min(floor_log2(ty ? ty : 1), TUNE_CY - 1);
+  *cp = planes <= 1 ? 0 : planes < 4 ? 1 : planes < 16 ? 2 : 3;
 }
 
 // Built-in choice: narrow images (fewer than 8 tile columns, so no XCD column
 // groups) with at least 4096 rows take 8-wave tiles of twice the height,
 // which halve the vertical carry words per pixel; everything else the 4-wave
-// tile.
-static int builtin_cfg(int cx, int cy) { return (cx < 3 && cy >= 6) ? 1 : 0; }
+// tile. The autotuner's table overrides it per class.
+static int builtin_cfg(int cx, int cy, int cp) {
+  (void)cp;
+  return (cx < 3 && cy >= 6) ? 1 : 0;
+}
+
+static bool class_ok(int out_bytes, int cx, int cy, int cp) {
+  return (out_bytes == 4 || out_bytes == 8) && cx >= 0 && cx < TUNE_CX && cy >= 0 && cy < TUNE_CY && cp >= 0 &&
+         cp < TUNE_CP;
+}
+
+static int tune_lookup(int out_bytes, int cx, int cy, int cp) {
+  const int v = g_tune[out_bytes == 8 ? 1 : 0][cx][cy][cp].load(std::memory_order_relaxed);
+  return v > 0 ? v - 1 : builtin_cfg(cx, cy, cp);
+}
 
 static int tune_pick(size_t out_size, uint32_t width, uint32_t height, uint32_t planes) {
-  int cx, cy;
-  tune_class(width, height, planes, &cx, &cy);
-  const int v = g_tune[out_size == 8 ? 1 : 0][cx][cy].load(std::memory_order_relaxed);
-  return v > 0 ? v - 1 : builtin_cfg(cx, cy);
+  int cx, cy, cp;
+  tune_class(width, height, planes, &cx, &cy, &cp);
+  return tune_lookup(out_size == 8 ? 8 : 4, cx, cy, cp);
 }
 
 static int batch_cfg_pick(size_t out_size) {
@@ -491,25 +506,22 @@ const char* sat_status_string(int status) {
 
 int sat_tile_config_count(void) { return TILE_CFGS; }
 
-void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class_x, int* class_y) {
-  int cx, cy;
-  tune_class(width, height, planes, &cx, &cy);
+void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class_x, int* class_y, int* class_p) {
+  int cx, cy, cp;
+  tune_class(width, height, planes, &cx, &cy, &cp);
   if (class_x) *class_x = cx;
   if (class_y) *class_y = cy;
+  if (class_p) *class_p = cp;
 }
 
-int sat_tune_get(int out_bytes, int class_x, int class_y) {
-  if ((out_bytes != 4 && out_bytes != 8) || class_x < 0 || class_x >= TUNE_CX || class_y < 0 || class_y >= TUNE_CY)
-    return -SAT_ERR_INVALID;
-  const int v = g_tune[out_bytes == 8 ? 1 : 0][class_x][class_y].load(std::memory_order_relaxed);
-  return v > 0 ? v - 1 : builtin_cfg(class_x, class_y);
+int sat_tune_get(int out_bytes, int class_x, int class_y, int class_p) {
+  if (!class_ok(out_bytes, class_x, class_y, class_p)) return -SAT_ERR_INVALID;
+  return tune_lookup(out_bytes, class_x, class_y, class_p);
 }
 
-int sat_tune_set(int out_bytes, int class_x, int class_y, int cfg) {
-  if ((out_bytes != 4 && out_bytes != 8) || class_x < 0 || class_x >= TUNE_CX || class_y < 0 || class_y >= TUNE_CY ||
-      cfg < -1 || cfg >= TILE_CFGS)
-    return SAT_ERR_INVALID;
-  g_tune[out_bytes == 8 ? 1 : 0][class_x][class_y].store(cfg + 1, std::memory_order_relaxed);
+int sat_tune_set(int out_bytes, int class_x, int class_y, int class_p, int cfg) {
+  if (!class_ok(out_bytes, class_x, class_y, class_p) || cfg < -1 || cfg >= TILE_CFGS) return SAT_ERR_INVALID;
+  g_tune[out_bytes == 8 ? 1 : 0][class_x][class_y][class_p].store(cfg + 1, std::memory_order_relaxed);
   return SAT_OK;
 }
 

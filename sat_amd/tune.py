@@ -40,9 +40,19 @@ QUICK_SHAPES = [(8192, 8192, 1), (65536, 256, 1), (1024, 1024, 96)]
 
 
 def classes_of(L, width: int, height: int, planes: int):
-    cx, cy = C.c_int(), C.c_int()
-    L.sat_tune_class(width, height, planes, C.byref(cx), C.byref(cy))
-    return cx.value, cy.value
+    cx, cy, cp = C.c_int(), C.c_int(), C.c_int()
+    L.sat_tune_class(width, height, planes, C.byref(cx), C.byref(cy), C.byref(cp))
+    return cx.value, cy.value, cp.value
+
+
+def reset_table(L) -> None:
+    """Every class back to the library's built-in choice."""
+    for ob in (4, 8):
+        L.sat_batch_tune_set(ob, -1)
+        for cx in range(9):
+            for cy in range(12):
+                for cp in range(4):
+                    L.sat_tune_set(ob, cx, cy, cp, -1)
 
 
 def apply_table(path) -> int:
@@ -54,7 +64,8 @@ def apply_table(path) -> int:
     doc = json.loads(Path(path).read_text())
     n = 0
     for e in doc.get("entries", []):
-        if L.sat_tune_set(int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]), int(e["cfg"])) == 0:
+        if L.sat_tune_set(int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]), int(e["class_p"]),
+                          int(e["cfg"])) == 0:
             n += 1
     for e in doc.get("batch", []):
         if L.sat_batch_tune_set(int(e["out_bytes"]), int(e["cfg"])) == 0:
@@ -86,7 +97,7 @@ def _tune_shape(L, sa, H, W, P, out_bytes, reps, log):
 
     dev = torch.device("cuda:0")
     g = torch.Generator(device="cpu").manual_seed(H * 131 + W * 7 + P)
-    cx, cy = classes_of(L, W, H, P)
+    cx, cy, cp = classes_of(L, W, H, P)
     if out_bytes == 4:
         # timing on f32, invariance on u8 -> u32 (bit-exact in every configuration)
         x = torch.rand((P, H, W), generator=g).to(dev)
@@ -100,7 +111,7 @@ def _tune_shape(L, sa, H, W, P, out_bytes, reps, log):
     yi = torch.empty(x.shape, dtype=int_dt, device=dev)
     times, ref, ref_f = [], None, None
     for cfg in range(L.sat_tile_config_count()):
-        sa.check(L.sat_tune_set(out_bytes, cx, cy, cfg), "sat_tune_set")
+        sa.check(L.sat_tune_set(out_bytes, cx, cy, cp, cfg), "sat_tune_set")
         assert L.sat_tune_pick(out_bytes, W, H, P) == cfg
         sa.integral(xi, out_dtype=int_dt, out=yi)
         sa.integral(x, out=y)
@@ -118,10 +129,10 @@ def _tune_shape(L, sa, H, W, P, out_bytes, reps, log):
                 raise AssertionError(f"tile configuration {cfg} differs by {err:.2e} (relative) at {P}x{H}x{W}")
         times.append(_time(lambda: sa.integral(x, out=y), reps))
     best = min(range(len(times)), key=lambda c: times[c])
-    log(f"[tune] {P}x{H}x{W} out {out_bytes}B class ({cx},{cy}): "
+    log(f"[tune] {P}x{H}x{W} out {out_bytes}B class ({cx},{cy},{cp}): "
         + "  ".join(f"cfg{c} {t:8.1f} us" for c, t in enumerate(times)) + f"  -> cfg{best}")
-    return {"out_bytes": out_bytes, "class_x": cx, "class_y": cy, "cfg": best, "shape": [H, W], "planes": P,
-            "us": [round(t, 2) for t in times]}
+    return {"out_bytes": out_bytes, "class_x": cx, "class_y": cy, "class_p": cp, "cfg": best, "shape": [H, W],
+            "planes": P, "us": [round(t, 2) for t in times]}
 
 
 def _tune_batch(L, sa, out_bytes, n, reps, log):
@@ -162,7 +173,7 @@ def _tune_batch(L, sa, out_bytes, n, reps, log):
     return {"out_bytes": out_bytes, "cfg": best, "images": n, "us": [round(t, 2) for t in times]}
 
 
-def tune(shapes=None, out_bytes=(4, 8), batch_images=256, reps=7, log=print) -> dict:
+def tune(shapes=None, out_bytes=(4, 8), batch_images=512, reps=7, log=print) -> dict:
     """Measure every tile configuration per shape class on cuda:0 and return the
     table (classes not covered keep the built-in choice)."""
     import torch
@@ -171,25 +182,31 @@ def tune(shapes=None, out_bytes=(4, 8), batch_images=256, reps=7, log=print) -> 
 
     L = sa.lib()
     shapes = FULL_SHAPES if shapes is None else shapes
-    entries = {}
+    measured = {}
     try:
         for ob in out_bytes:
             for (H, W, P) in shapes:
                 if ob == 8 and H * W * P > (1 << 28):
                     continue  # 8-byte outputs: keep each sweep image under 2 GiB
                 e = _tune_shape(L, sa, H, W, P, ob, reps, log)
-                entries[(ob, e["class_x"], e["class_y"])] = e  # the last shape of a class decides
+                measured.setdefault((ob, e["class_x"], e["class_y"], e["class_p"]), []).append(e)
         batch = [_tune_batch(L, sa, ob, batch_images, reps, log) for ob in out_bytes] if batch_images else []
     finally:
-        for ob in (4, 8):
-            L.sat_batch_tune_set(ob, -1)
-            for cx in range(9):
-                for cy in range(12):
-                    L.sat_tune_set(ob, cx, cy, -1)
+        reset_table(L)
     torch.cuda.synchronize()
+    entries = {}
+    for key, es in measured.items():
+        # several shapes in one class: the configuration with the least summed
+        # slowdown relative to each shape's best
+        ncfg = len(es[0]["us"])
+        cost = [sum(e["us"][c] / min(e["us"]) for e in es) for c in range(ncfg)]
+        best = min(range(ncfg), key=lambda c: cost[c])
+        entries[key] = {"out_bytes": key[0], "class_x": key[1], "class_y": key[2], "class_p": key[3], "cfg": best,
+                        "shapes": [[e["planes"], e["shape"][0], e["shape"][1]] for e in es],
+                        "us": [e["us"] for e in es]}
     return {"arch": torch.cuda.get_device_properties(0).gcnArchName.split(":")[0],
             "generated_by": "python -m sat_amd.tune", "configs": CFG_NAMES,
-            "entries": sorted(entries.values(), key=lambda e: (e["out_bytes"], e["class_x"], e["class_y"])),
+            "entries": [entries[k] for k in sorted(entries)],
             "batch": batch}
 
 

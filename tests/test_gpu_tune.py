@@ -29,17 +29,15 @@ def sa():
 
     L = sat_amd.lib()
     yield sat_amd
-    for ob in (4, 8):
-        L.sat_batch_tune_set(ob, -1)
-        for cx in range(9):
-            for cy in range(12):
-                L.sat_tune_set(ob, cx, cy, -1)
+    from sat_amd.tune import reset_table
+
+    reset_table(L)
 
 
 def force(L, out_bytes, w, h, p, cfg):
-    cx, cy = C.c_int(), C.c_int()
-    L.sat_tune_class(w, h, p, C.byref(cx), C.byref(cy))
-    assert L.sat_tune_set(out_bytes, cx.value, cy.value, cfg) == 0
+    cx, cy, cp = C.c_int(), C.c_int(), C.c_int()
+    L.sat_tune_class(w, h, p, C.byref(cx), C.byref(cy), C.byref(cp))
+    assert L.sat_tune_set(out_bytes, cx.value, cy.value, cp.value, cfg) == 0
     assert L.sat_tune_pick(out_bytes, w, h, p) == cfg
 
 

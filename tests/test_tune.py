@@ -16,25 +16,24 @@ def L():
 
     L = sat_amd.lib()
     yield L
-    for ob in (4, 8):
-        L.sat_batch_tune_set(ob, -1)
-        for cx in range(9):
-            for cy in range(12):
-                L.sat_tune_set(ob, cx, cy, -1)
+    from sat_amd.tune import reset_table
+
+    reset_table(L)
 
 
 def py_class(w, h, p):
-    tx, ty = math.ceil(w / 256), math.ceil(h * p / 64)
-    return min(int(math.floor(math.log2(tx))), 8), min(int(math.floor(math.log2(ty))), 11)
+    tx, ty = math.ceil(w / 256), math.ceil(h / 64)
+    cp = 0 if p <= 1 else 1 if p < 4 else 2 if p < 16 else 3
+    return min(int(math.floor(math.log2(tx))), 8), min(int(math.floor(math.log2(ty))), 11), cp
 
 
 def test_class_formula(L):
     rng = np.random.default_rng(5)
     for _ in range(500):
         w, h, p = int(rng.integers(1, 1 << 17)), int(rng.integers(1, 1 << 17)), int(rng.integers(1, 64))
-        cx, cy = C.c_int(), C.c_int()
-        L.sat_tune_class(w, h, p, C.byref(cx), C.byref(cy))
-        assert (cx.value, cy.value) == py_class(w, h, p), (w, h, p)
+        cx, cy, cp = C.c_int(), C.c_int(), C.c_int()
+        L.sat_tune_class(w, h, p, C.byref(cx), C.byref(cy), C.byref(cp))
+        assert (cx.value, cy.value, cp.value) == py_class(w, h, p), (w, h, p)
 
 
 def test_builtin_choice(L):
@@ -48,23 +47,25 @@ def test_builtin_choice(L):
 
 
 def test_set_get_roundtrip(L):
-    assert L.sat_tune_set(4, 5, 7, 2) == 0
-    assert L.sat_tune_get(4, 5, 7) == 2
-    assert L.sat_tune_pick(4, 8192, 8192, 1) == 2  # class (5, 7)
+    assert L.sat_tune_set(4, 5, 7, 0, 2) == 0
+    assert L.sat_tune_get(4, 5, 7, 0) == 2
+    assert L.sat_tune_pick(4, 8192, 8192, 1) == 2  # class (5, 7, 0)
+    assert L.sat_tune_pick(4, 8192, 8192, 2) == 0  # other plane class untouched
     assert L.sat_tune_pick(8, 8192, 8192, 1) == 0  # other output width untouched
-    assert L.sat_tune_set(4, 5, 7, -1) == 0
-    assert L.sat_tune_get(4, 5, 7) == 0
+    assert L.sat_tune_set(4, 5, 7, 0, -1) == 0
+    assert L.sat_tune_get(4, 5, 7, 0) == 0
     assert L.sat_batch_tune_set(8, 1) == 0 and L.sat_batch_tune_get(8) == 1
     assert L.sat_batch_tune_set(8, -1) == 0 and L.sat_batch_tune_get(8) == 0
 
 
 def test_invalid_arguments(L):
-    assert L.sat_tune_set(2, 0, 0, 0) != 0
-    assert L.sat_tune_set(4, 9, 0, 0) != 0
-    assert L.sat_tune_set(4, 0, 12, 0) != 0
-    assert L.sat_tune_set(4, 0, 0, 3) != 0
-    assert L.sat_tune_set(4, -1, 0, 0) != 0
-    assert L.sat_tune_get(16, 0, 0) < 0
+    assert L.sat_tune_set(2, 0, 0, 0, 0) != 0
+    assert L.sat_tune_set(4, 9, 0, 0, 0) != 0
+    assert L.sat_tune_set(4, 0, 12, 0, 0) != 0
+    assert L.sat_tune_set(4, 0, 0, 4, 0) != 0
+    assert L.sat_tune_set(4, 0, 0, 0, 3) != 0
+    assert L.sat_tune_set(4, -1, 0, 0, 0) != 0
+    assert L.sat_tune_get(16, 0, 0, 0) < 0
     assert L.sat_batch_tune_set(4, 7) != 0
 
 
@@ -72,8 +73,7 @@ def test_workspace_fits_every_config(L):
     # workspace queries do not depend on the table
     before = L.sat_integral_workspace_bytes(3, 6, 6, 1000, 70000)
     for cfg in range(3):
-        cx, cy = py_class(1000, 70000, 1)
-        L.sat_tune_set(4, cx, cy, cfg)
+        L.sat_tune_set(4, *py_class(1000, 70000, 1), cfg)
         assert L.sat_integral_workspace_bytes(3, 6, 6, 1000, 70000) == before
     L.sat_tune_set(4, *py_class(1000, 70000, 1), -1)
 
@@ -81,13 +81,13 @@ def test_workspace_fits_every_config(L):
 def test_apply_table(L, tmp_path):
     from sat_amd.tune import apply_table
 
-    doc = {"entries": [{"out_bytes": 4, "class_x": 2, "class_y": 10, "cfg": 2},
-                       {"out_bytes": 8, "class_x": 0, "class_y": 0, "cfg": 1}],
+    doc = {"entries": [{"out_bytes": 4, "class_x": 2, "class_y": 10, "class_p": 0, "cfg": 2},
+                       {"out_bytes": 8, "class_x": 0, "class_y": 0, "class_p": 3, "cfg": 1}],
            "batch": [{"out_bytes": 4, "cfg": 2}]}
     p = tmp_path / "t.json"
     p.write_text(json.dumps(doc))
     assert apply_table(p) == 3
-    assert L.sat_tune_get(4, 2, 10) == 2 and L.sat_tune_get(8, 0, 0) == 1 and L.sat_batch_tune_get(4) == 2
-    L.sat_tune_set(4, 2, 10, -1)
-    L.sat_tune_set(8, 0, 0, -1)
+    assert L.sat_tune_get(4, 2, 10, 0) == 2 and L.sat_tune_get(8, 0, 0, 3) == 1 and L.sat_batch_tune_get(4) == 2
+    L.sat_tune_set(4, 2, 10, 0, -1)
+    L.sat_tune_set(8, 0, 0, 3, -1)
     L.sat_batch_tune_set(4, -1)
