@@ -36,7 +36,20 @@ def test_class_formula(L):
         assert (cx.value, cy.value, cp.value) == py_class(w, h, p), (w, h, p)
 
 
+def test_tuned_table_loaded(L):
+    """The package applies sat_amd/tuned_gfx950.json when it loads the library."""
+    import sat_amd
+
+    doc = json.loads(sat_amd.TUNED_TABLE.read_text())
+    assert doc["entries"]
+    for e in doc["entries"]:
+        assert L.sat_tune_get(e["out_bytes"], e["class_x"], e["class_y"], e["class_p"]) == e["cfg"]
+
+
 def test_builtin_choice(L):
+    from sat_amd.tune import reset_table
+
+    reset_table(L)
     assert L.sat_tile_config_count() == 3
     # narrow and tall: 8-wave double-height tiles; square and wide: 4-wave tiles
     assert L.sat_tune_pick(4, 256, 65536, 1) == 1
