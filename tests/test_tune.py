@@ -5,6 +5,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -36,14 +37,25 @@ def test_class_formula(L):
         assert (cx.value, cy.value, cp.value) == py_class(w, h, p), (w, h, p)
 
 
-def test_tuned_table_loaded(L):
-    """The package applies sat_amd/tuned_gfx950.json when it loads the library."""
-    import sat_amd
+def test_tuned_table_loaded():
+    """The package applies sat_amd/tuned_gfx950.json when it loads the library
+    (checked in a fresh process: other tests reset the table)."""
+    import subprocess
+    import sys
 
-    doc = json.loads(sat_amd.TUNED_TABLE.read_text())
-    assert doc["entries"]
-    for e in doc["entries"]:
-        assert L.sat_tune_get(e["out_bytes"], e["class_x"], e["class_y"], e["class_p"]) == e["cfg"]
+    code = (
+        "import json, sat_amd\n"
+        "L = sat_amd.lib()\n"
+        "doc = json.loads(sat_amd.TUNED_TABLE.read_text())\n"
+        "assert doc['entries']\n"
+        "for e in doc['entries']:\n"
+        "    assert L.sat_tune_get(e['out_bytes'], e['class_x'], e['class_y'], e['class_p']) == e['cfg'], e\n"
+        "for e in doc['batch']:\n"
+        "    assert L.sat_batch_tune_get(e['out_bytes']) == e['cfg'], e\n"
+        "print('table ok')\n")
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "table ok" in r.stdout, r.stderr[-2000:]
 
 
 def test_builtin_choice(L):
