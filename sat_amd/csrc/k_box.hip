@@ -50,14 +50,96 @@ template <typename T> struct GradCfg;
 template <> struct GradCfg<float> { static constexpr int NW = 4, RPT = 8; };
 template <> struct GradCfg<double> { static constexpr int NW = 4, RPT = 4; };
 
+// Corner impulses of the boxes binned by the tile that holds them: one
+// workgroup per plane counts the impulses per tile in LDS, scans the counts
+// into offsets and places (position in tile, value) pairs, so that a gradient
+// tile reads only its own impulses. off[p*(tpp+1) + i] is the first entry of
+// physical tile i of plane p (entries of plane p start at p*4*nbox).
+constexpr int BIN_THREADS = 1024;
+constexpr uint32_t BIN_MAX_TILES = 8191;  // LDS counters per plane
+
+template <typename T>
+__global__ void __launch_bounds__(BIN_THREADS)
+    box_bin_kernel(const int4* __restrict__ boxes, const T* __restrict__ gout, uint32_t nbox, uint32_t W, uint32_t H,
+                   uint32_t nTx, uint32_t tpp, uint32_t* __restrict__ off, uint32_t* __restrict__ pos,
+                   T* __restrict__ val) {
+  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  constexpr int TH = Op::TH, TW = Op::TW;
+  __shared__ uint32_t cnt[BIN_MAX_TILES + 1];
+  __shared__ uint32_t part[BIN_THREADS];
+  const uint32_t p = blockIdx.x, tid = threadIdx.x;
+  const int4* pb = boxes + (uint64_t)p * nbox;
+  const T* pg = gout + (uint64_t)p * nbox;
+  for (uint32_t i = tid; i <= tpp; i += BIN_THREADS) cnt[i] = 0;
+  __syncthreads();
+  // pass 1: impulses per tile
+  for (uint32_t b = tid; b < nbox; b += BIN_THREADS) {
+    const int4 bx = pb[b];
+    if (!box_ok(bx, W, H)) continue;
+    const int ys[2] = {bx.z, bx.x - 1}, xs[2] = {bx.w, bx.y - 1};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        if (ys[a] >= 0 && xs[c] >= 0) atomicAdd(&cnt[(uint32_t)ys[a] / TH * nTx + (uint32_t)xs[c] / TW], 1u);
+  }
+  __syncthreads();
+  // exclusive scan of cnt[0..tpp): each thread folds a run, then the runs
+  const uint32_t per = (tpp + BIN_THREADS - 1) / BIN_THREADS, i0 = tid * per;
+  uint32_t run = 0;
+  for (uint32_t i = i0; i < i0 + per && i < tpp; ++i) run += cnt[i];
+  part[tid] = run;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < BIN_THREADS; ++k) {
+      const uint32_t v = part[k];
+      part[k] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  uint32_t acc = part[tid];
+  for (uint32_t i = i0; i < i0 + per && i < tpp; ++i) {
+    const uint32_t v = cnt[i];
+    cnt[i] = acc;
+    off[(uint64_t)p * (tpp + 1) + i] = acc;
+    acc += v;
+  }
+  if (i0 < tpp && i0 + per >= tpp) off[(uint64_t)p * (tpp + 1) + tpp] = acc;
+  if (tpp == 0 && tid == 0) off[(uint64_t)p * (tpp + 1)] = 0;
+  __syncthreads();
+  // pass 2: place the impulses (cnt now holds each tile's cursor)
+  uint32_t* ppos = pos + (uint64_t)p * 4 * nbox;
+  T* pval = val + (uint64_t)p * 4 * nbox;
+  for (uint32_t b = tid; b < nbox; b += BIN_THREADS) {
+    const int4 bx = pb[b];
+    if (!box_ok(bx, W, H)) continue;
+    const T gv = pg[b];
+    const int ys[2] = {bx.z, bx.x - 1}, xs[2] = {bx.w, bx.y - 1};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        if (ys[a] >= 0 && xs[c] >= 0) {
+          const uint32_t ty = (uint32_t)ys[a] / TH, tx = (uint32_t)xs[c] / TW;
+          const uint32_t slot = atomicAdd(&cnt[ty * nTx + tx], 1u);
+          ppos[slot] = ((uint32_t)ys[a] - ty * TH) * TW + ((uint32_t)xs[c] - tx * TW);
+          pval[slot] = (a ^ c) ? -gv : gv;
+        }
+  }
+}
+
 // grad_x = reverse SAT of the boxes' corner impulses (+g at (y1,x1) and
-// (y0-1,x0-1), -g at (y0-1,x1) and (y1,x0-1)). Each tile rebuilds its slice of
-// the impulse image in LDS from the plane's box list instead of reading a
-// dense impulse image, then runs the reverse-SAT tile operator.
+// (y0-1,x0-1), -g at (y0-1,x1) and (y1,x0-1)). Each tile builds its slice of
+// the impulse image in LDS (from its binned impulses when `off` is given,
+// otherwise from the plane's whole box list) instead of reading a dense
+// impulse image, then runs the reverse-SAT tile operator.
 template <typename T>
 __global__ void __launch_bounds__(64 * GradCfg<T>::NW)
     box_grad_kernel(const int4* __restrict__ boxes, const T* __restrict__ gout, uint32_t nbox, ImgArgs g,
-                    uint32_t tpp, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr) {
+                    uint32_t tpp, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr,
+                    const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos, const T* __restrict__ val) {
   using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
   constexpr int TH = Op::TH, TW = Op::TW, NT = Op::NT, RPT = GradCfg<T>::RPT;
   union Smem {
@@ -78,7 +160,14 @@ __global__ void __launch_bounds__(64 * GradCfg<T>::NW)
   __syncthreads();
   const int4* pb = boxes + (uint64_t)p * nbox;
   const T* pg = gout + (uint64_t)p * nbox;
-  for (uint32_t b = threadIdx.x; b < nbox; b += NT) {
+  if (off) {
+    const uint32_t pt = (uint32_t)(k.py0 / TH) * g.nTx + (uint32_t)(k.px0 / TW);  // physical tile
+    const uint32_t* po = off + (uint64_t)p * (tpp + 1);
+    const uint64_t base = (uint64_t)p * 4 * nbox;
+    const uint32_t e1 = po[pt + 1];
+    for (uint32_t e = po[pt] + threadIdx.x; e < e1; e += NT) atomicAdd(&sm.imp[pos[base + e]], val[base + e]);
+  }
+  for (uint32_t b = threadIdx.x; !off && b < nbox; b += NT) {
     const int4 bx = pb[b];
     if (!box_ok(bx, g.W, g.H)) continue;
     const T gv = pg[b];
@@ -161,9 +250,25 @@ size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t
 }
 
 template <typename T>
+static uint64_t grad_tiles_per_plane(uint32_t width, uint32_t height) {
+  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  return (uint64_t)((width + Op::TW - 1) / Op::TW) * ((height + Op::TH - 1) / Op::TH);
+}
+
+// Bytes of the impulse bins (0: the shape takes the unbinned path).
+size_t box_grad_bin_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes, uint32_t nbox) {
+  const uint64_t tpp = dtype == SAT_F64 ? grad_tiles_per_plane<double>(width, height)
+                                        : grad_tiles_per_plane<float>(width, height);
+  if (tpp == 0 || tpp > BIN_MAX_TILES || (uint64_t)nbox * 4 > 0xffffffffull) return 0;
+  const uint64_t n = (uint64_t)planes * nbox * 4;
+  const uint64_t off_b = (planes * (tpp + 1) * 4 + 255) / 256 * 256, pos_b = (n * 4 + 255) / 256 * 256;
+  return (size_t)(off_b + pos_b + n * (dtype == SAT_F64 ? 8 : 4));
+}
+
+template <typename T>
 static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
                               int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes, void* ws,
-                              size_t ws_bytes, uint32_t epoch, hipStream_t stream) {
+                              size_t ws_bytes, void* bins, uint32_t epoch, hipStream_t stream) {
   using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
   const uint64_t nTx = (width + Op::TW - 1) / Op::TW, nTy = (height + Op::TH - 1) / Op::TH;
   const uint64_t tpp = nTx * nTy, ntiles = tpp * planes;
@@ -186,20 +291,32 @@ static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* g
   g.vin = 0;
   g.vout = aligned_for(grad_x, pitch, plane_stride, sizeof(T)) ? 1u : 0u;
   g.fault = &((WsHeader*)ws)->fault;
+  uint32_t *off = nullptr, *pos = nullptr;
+  T* val = nullptr;
+  if (bins && nbox > 0) {
+    const uint64_t n = (uint64_t)planes * nbox * 4;
+    const uint64_t off_b = (planes * (tpp + 1) * 4 + 255) / 256 * 256, pos_b = (n * 4 + 255) / 256 * 256;
+    off = (uint32_t*)bins;
+    pos = (uint32_t*)((char*)bins + off_b);
+    val = (T*)((char*)bins + off_b + pos_b);
+    hipLaunchKernelGGL(box_bin_kernel<T>, dim3(planes), dim3(BIN_THREADS), 0, stream, (const int4*)boxes,
+                       (const T*)grad_out, nbox, width, height, (uint32_t)nTx, (uint32_t)tpp, off, pos, val);
+  }
   hipLaunchKernelGGL(box_grad_kernel<T>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, (const int4*)boxes,
-                     (const T*)grad_out, nbox, g, (uint32_t)tpp, plane_stride, epoch, (const WsHeader*)ws);
+                     (const T*)grad_out, nbox, g, (uint32_t)tpp, plane_stride, epoch, (const WsHeader*)ws,
+                     (const uint32_t*)off, (const uint32_t*)pos, (const T*)val);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 
 sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x,
                            int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
-                           void* ws, size_t ws_bytes, uint32_t epoch, hipStream_t stream) {
+                           void* ws, size_t ws_bytes, void* bins, uint32_t epoch, hipStream_t stream) {
   if (dtype == SAT_F32)
     return grad_launch<float>(boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, ws, ws_bytes,
-                              epoch, stream);
+                              bins, epoch, stream);
   if (dtype == SAT_F64)
     return grad_launch<double>(boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, ws, ws_bytes,
-                               epoch, stream);
+                               bins, epoch, stream);
   return SAT_ERR_DTYPE;
 }
 

@@ -749,10 +749,13 @@ int sat_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, const void* 
                             (hipStream_t)stream);
 }
 
+// Workspace of sat_box_grad: header and carries, then (256-aligned) the
+// impulse bins, which never share the carry area.
 size_t sat_box_grad_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes, uint32_t nbox) {
-  (void)nbox;
   if (dtype != SAT_F32 && dtype != SAT_F64) return 0;
-  return box_grad_carry_bytes(dtype, width, height, planes);
+  const size_t carry = box_grad_carry_bytes(dtype, width, height, planes);
+  const size_t bins = box_grad_bin_bytes(dtype, width, height, planes, nbox);
+  return bins ? (size_t)round_up((int64_t)carry, 256) + bins : carry;
 }
 
 int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
@@ -765,13 +768,25 @@ int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* gra
   if (planes > 1 && plane_stride < pitch * (int64_t)height) return SAT_ERR_INVALID;
   hipStream_t st = (hipStream_t)stream;
   const size_t need = box_grad_carry_bytes(dtype, width, height, planes);
+  const size_t bin_bytes = box_grad_bin_bytes(dtype, width, height, planes, nbox);
+  void* bins = nullptr;
+  if (workspace) {
+    if (bin_bytes) {
+      const size_t at = (size_t)round_up((int64_t)need, 256);
+      if (workspace_bytes < at + bin_bytes) return SAT_ERR_WORKSPACE;
+      bins = (char*)workspace + at;
+    }
+  } else if (bin_bytes) {
+    const sat_status s = managed_buf(st, WS_TMP, bin_bytes, &bins);
+    if (s != SAT_OK) return s;
+  }
   CarryCtx c;
-  sat_status s = carry_open(st, workspace, workspace_bytes, need, &c);
+  sat_status s = carry_open(st, workspace, workspace ? need : 0, need, &c);
   if (s != SAT_OK) return s;
   uint32_t ep = 0;
   if ((s = carry_epoch(c, st, &ep)) != SAT_OK) return s;
   return launch_box_grad(dtype, boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, c.ws,
-                         c.bytes, ep, st);
+                         c.bytes, bins, ep, st);
 }
 
 }  // extern "C"

@@ -72,7 +72,9 @@ sat_status launch_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, co
                               hipStream_t stream);
 sat_status launch_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x,
                            int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes,
-                           void* ws, size_t ws_bytes, uint32_t epoch, hipStream_t stream);
+                           void* ws, size_t ws_bytes, void* bins, uint32_t epoch, hipStream_t stream);
+// impulse bins of the fused box gradient (0: that shape scans the box list)
+size_t box_grad_bin_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes, uint32_t nbox);
 size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes);
 
 // Zero `bytes` (rounded down to 8) at `p` with a fill kernel (k_sat.hip).

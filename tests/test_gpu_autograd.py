@@ -155,3 +155,32 @@ def test_managed_path_graph_replay(O, dev, sa):
         assert np.array_equal(y.view(torch.int32).cpu().numpy().view(np.uint32), exp)
         assert np.array_equal(z.view(torch.int32).cpu().numpy().view(np.uint32), exp)
     assert sa.lib().sat_workspace_fault(None, s.cuda_stream) == 0
+
+
+def test_box_grad_caller_workspace_many_planes(O, dev, sa):
+    """The binned fused gradient (caller-provided workspace, many planes,
+    boxes touching the borders) equals reverse SAT of the scattered corner
+    impulses, both in f64."""
+    import ctypes as C
+
+    say("[case] box_grad binned, caller workspace, 24 planes")
+    P, H, W, nbox = 24, 300, 520, 700
+    boxes_np = make_boxes(P, nbox, H, W, 11)
+    boxes_np[:, 0] = [0, 0, H - 1, W - 1]  # whole image
+    boxes_np[:, 1] = [H - 1, W - 1, H - 1, W - 1]  # last pixel
+    boxes = torch.from_numpy(boxes_np).to(dev)
+    g = torch.from_numpy(O.gen("f64n", (P, nbox), 12)).to(dev)
+    L = sa.lib()
+    nb = L.sat_box_grad_workspace_bytes(7, W, H, P, nbox)
+    assert nb > L.sat_box_grad_workspace_bytes(7, W, H, P, 0)  # room for the bins
+    ws = torch.full((nb,), 0xAB, dtype=torch.uint8, device=dev)  # garbage: the library clears what it needs
+    gx = torch.empty((P, H, W), dtype=torch.float64, device=dev)
+    for rep in range(2):
+        st = L.sat_box_grad(7, boxes.data_ptr(), nbox, g.data_ptr(), gx.data_ptr(), W, W * H, W, H, P,
+                            ws.data_ptr(), nb, torch.cuda.current_stream().cuda_stream)
+        assert st == 0
+        gs = sa.box_scatter(boxes, g, H, W)
+        ref = sa.integral_reverse(gs)
+        torch.cuda.synchronize()
+        assert torch.allclose(gx, ref, rtol=0, atol=1e-9), float((gx - ref).abs().max())
+    assert L.sat_workspace_fault(C.c_void_p(ws.data_ptr()), None) == 0
