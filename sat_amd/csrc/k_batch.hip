@@ -1,7 +1,8 @@
 // Batched SAT: one launch walks a queue of (image, tile) items over images of
-// uneven sizes and mixed dtype pairs. Queue positions are image-major and
-// row-major inside an image, so the look-back of a tile only ever waits for
-// tiles of its own image that precede it in the queue.
+// uneven sizes and mixed dtype pairs. The host deals whole images to XCDs and
+// keeps each image's tiles row-major and in increasing queue position, so the
+// look-back of a tile only ever waits for tiles of its own image, on its own
+// XCD, that precede it in the queue.
 #include "sat_launch.hpp"
 
 namespace satdev {
@@ -26,24 +27,24 @@ __device__ __forceinline__ void run_kind(const ImgArgs& g, uint32_t lt, uint32_t
   Op::run(g, lt, ep4, *reinterpret_cast<typename Op::Smem*>(smem));
 }
 
-// One workgroup per queue item: item t = blockIdx.x is tile t - tile_base of
-// image tile_img[t]. The launch epoch is the workspace's device epoch,
-// advanced by bump_epoch_kernel before every launch, so the same launch can
-// be replayed from a captured graph.
+// One workgroup per queue item: item b = blockIdx.x is tile queue[2b+1] of
+// image queue[2b] (an empty item returns at once). The launch epoch is the
+// workspace's device epoch, advanced by bump_epoch_kernel before every
+// launch, so the same launch can be replayed from a captured graph.
 template <int OUTB, int CFG>
 __global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW)) __attribute__((amdgpu_waves_per_eu(4)))
     batch_kernel(const WsHeader* hdr, const sathost::BatchItem* __restrict__ items,
-                 const uint32_t* __restrict__ tile_img) {
+                 const uint32_t* __restrict__ queue) {
   using Cfg = BatchCfg<OUTB, CFG>;
   using Ref = typename Cfg::Ref;
   using RefOp = TileOp<Ref, Ref, MODE_SAT, false, false, 0, Cfg::NW, Cfg::RPT>;
   __shared__ typename RefOp::Smem sm;
   const uint32_t ep4 = launch_ep4(0, hdr);
-  const uint32_t t = blockIdx.x;
-  const uint32_t img = __builtin_amdgcn_readfirstlane(tile_img[t]);
+  const uint32_t img = __builtin_amdgcn_readfirstlane(queue[2 * blockIdx.x]);
+  if (img == 0xffffffffu) return;
+  const uint32_t lt = __builtin_amdgcn_readfirstlane(queue[2 * blockIdx.x + 1]);
   const sathost::BatchItem& it = items[img];
   const ImgArgs g = it.g;
-  const uint32_t lt = t - it.tile_base;
   const uint32_t kind = __builtin_amdgcn_readfirstlane(it.kind);
   if constexpr (OUTB == 4) {
     switch (kind) {
@@ -102,23 +103,23 @@ Geometry batch_geometry(size_t out_size, int cfg) {
 }
 
 template <int OUTB, int CFG>
-static void launch_batch_cfg(const BatchItem* items, const uint32_t* tile_img, uint32_t total_tiles, void* ws,
+static void launch_batch_cfg(const BatchItem* items, const uint32_t* queue, uint32_t blocks, void* ws,
                              hipStream_t stream) {
-  hipLaunchKernelGGL((satdev::batch_kernel<OUTB, CFG>), dim3(total_tiles), dim3(64 * satdev::BatchCfg<OUTB, CFG>::NW),
-                     0, stream, (const satdev::WsHeader*)ws, items, tile_img);
+  hipLaunchKernelGGL((satdev::batch_kernel<OUTB, CFG>), dim3(blocks), dim3(64 * satdev::BatchCfg<OUTB, CFG>::NW),
+                     0, stream, (const satdev::WsHeader*)ws, items, queue);
 }
 
-sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* tile_img,
-                        uint32_t total_tiles, void* ws, hipStream_t stream) {
-  if (total_tiles == 0) return SAT_OK;
+sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* queue, uint32_t blocks,
+                        void* ws, hipStream_t stream) {
+  if (blocks == 0) return SAT_OK;
   if (out_size == 8) {
-    if (cfg == 1) launch_batch_cfg<8, 1>(items, tile_img, total_tiles, ws, stream);
-    else if (cfg == 2) launch_batch_cfg<8, 2>(items, tile_img, total_tiles, ws, stream);
-    else launch_batch_cfg<8, 0>(items, tile_img, total_tiles, ws, stream);
+    if (cfg == 1) launch_batch_cfg<8, 1>(items, queue, blocks, ws, stream);
+    else if (cfg == 2) launch_batch_cfg<8, 2>(items, queue, blocks, ws, stream);
+    else launch_batch_cfg<8, 0>(items, queue, blocks, ws, stream);
   } else {
-    if (cfg == 1) launch_batch_cfg<4, 1>(items, tile_img, total_tiles, ws, stream);
-    else if (cfg == 2) launch_batch_cfg<4, 2>(items, tile_img, total_tiles, ws, stream);
-    else launch_batch_cfg<4, 0>(items, tile_img, total_tiles, ws, stream);
+    if (cfg == 1) launch_batch_cfg<4, 1>(items, queue, blocks, ws, stream);
+    else if (cfg == 2) launch_batch_cfg<4, 2>(items, queue, blocks, ws, stream);
+    else launch_batch_cfg<4, 0>(items, queue, blocks, ws, stream);
   }
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }

@@ -288,8 +288,6 @@ static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* g
   g.nTy = (uint32_t)nTy;
   g.vcarry = carries;
   g.hcarry = carries + vw;
-  g.vin = 0;
-  g.vout = aligned_for(grad_x, pitch, plane_stride, sizeof(T)) ? 1u : 0u;
   g.fault = &((WsHeader*)ws)->fault;
   uint32_t *off = nullptr, *pos = nullptr;
   T* val = nullptr;

@@ -402,23 +402,54 @@ static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_
 
 // ---------------------------------------------------------------------------
 // Batch plans. Layout in the workspace:
-//   [0, 512)          launch-state header
+//   [0, 256)          launch-state header
 //   [256, items)      carry words
 //   items             BatchItem[count]
-//   tile_img          uint32_t[total_tiles]
+//   queue             uint32_t[2 * blocks]: (image, tile of the image) per workgroup
+//
+// Queue order: workgroup b runs on XCD b % nx and each XCD starts its
+// workgroups in increasing b, so images are dealt whole to XCDs (largest
+// first, each to the XCD with the fewest bytes so far): every look-back stays
+// on one die. An XCD walks its images tile row by tile row, first row of every
+// image, then the second rows, ...: a tile's upper neighbour was started a
+// whole round of rows earlier and has usually finished, so the column
+// look-back reads one predecessor. The XCDs' shares are interleaved (block
+// k*nx + x is the k-th item of XCD x) and padded with empty items to equal
+// length. Each image's tiles stay row-major and in increasing block order, so
+// the queue is also safe for any other dealing of workgroups that starts them
+// in increasing b.
 // ---------------------------------------------------------------------------
+constexpr uint32_t QUEUE_EMPTY = 0xffffffffu;
+
 struct BatchPlanInfo {
   int out_size = 0;
   int cfg = 0;
   uint32_t total_tiles = 0;
+  uint32_t blocks = 0;  // workgroups of the launch (tiles + empty items)
   uint64_t carry_words = 0;
-  size_t items_off = 0, tile_img_off = 0, total = 0;
+  size_t items_off = 0, queue_off = 0, total = 0;
 };
 static std::mutex g_batch_mu;
 static std::map<uintptr_t, BatchPlanInfo> g_batch;
 
+// Deal images to XCDs (see the layout comment): per-XCD lists of image ids,
+// balanced by bytes moved.
+static std::vector<std::vector<uint32_t>> deal_images(const std::vector<uint64_t>& bytes, int nx) {
+  std::vector<uint32_t> order(bytes.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return bytes[a] > bytes[b]; });
+  std::vector<std::vector<uint32_t>> lists(nx);
+  std::vector<uint64_t> load(nx, 0);
+  for (uint32_t i : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lists[x].push_back(i);
+    load[x] += bytes[i];
+  }
+  return lists;
+}
+
 static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPlanInfo* info,
-                               std::vector<BatchItem>* items, std::vector<uint32_t>* tile_img) {
+                               std::vector<BatchItem>* items, std::vector<uint32_t>* queue) {
   if (count < 0) return SAT_ERR_INVALID;
   if (count > 0 && !im) return SAT_ERR_NULL;
   int osz = 0;
@@ -435,18 +466,46 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
   if (cfg < 0) cfg = batch_cfg_pick(osz);
   const Geometry g = batch_geometry(osz, cfg);
   uint64_t tiles = 0;
-  for (int i = 0; i < count; ++i) tiles += tiles_of(g, im[i].width, im[i].height, 1);
+  std::vector<uint64_t> ntiles(count);
+  for (int i = 0; i < count; ++i) tiles += (ntiles[i] = tiles_of(g, im[i].width, im[i].height, 1));
   if (tiles > 0x7fffffffull) return SAT_ERR_INVALID;
+  const int nx = std::max(1, device_xcd_count());
+  std::vector<uint64_t> bytes(count);
+  for (int i = 0; i < count; ++i)
+    bytes[i] = (uint64_t)im[i].width * im[i].height * (dtype_size(im[i].tin) + dtype_size(im[i].tout));
+  const auto lists = deal_images(bytes, nx);
+  uint64_t longest = 0;
+  for (int x = 0; x < nx; ++x) {
+    uint64_t n = 0;
+    for (uint32_t i : lists[x]) n += ntiles[i];
+    longest = std::max(longest, n);
+  }
+  const uint64_t blocks = longest * (uint64_t)nx;
+  if (blocks > 0x7fffffffull) return SAT_ERR_INVALID;
   info->out_size = osz;
   info->cfg = cfg;
   info->total_tiles = (uint32_t)tiles;
+  info->blocks = (uint32_t)blocks;
   info->carry_words = tiles * (uint64_t)(g.vwords + g.hwords);
   info->items_off = (size_t)round_up(satdev::WS_HEADER_BYTES + (int64_t)info->carry_words * 8, 256);
-  info->tile_img_off = (size_t)round_up((int64_t)(info->items_off + sizeof(BatchItem) * (size_t)count), 256);
-  info->total = info->tile_img_off + sizeof(uint32_t) * (size_t)tiles;
+  info->queue_off = (size_t)round_up((int64_t)(info->items_off + sizeof(BatchItem) * (size_t)count), 256);
+  info->total = info->queue_off + 2 * sizeof(uint32_t) * (size_t)blocks;
   if (!items) return SAT_OK;
   items->resize(count);
-  tile_img->resize(tiles);
+  queue->assign(2 * blocks, QUEUE_EMPTY);
+  for (int x = 0; x < nx; ++x) {
+    uint64_t k = 0, rows = 0;
+    for (uint32_t i : lists[x]) rows = std::max<uint64_t>(rows, (im[i].height + g.th - 1) / g.th);
+    for (uint64_t r = 0; r < rows; ++r)
+      for (uint32_t i : lists[x]) {
+        const uint64_t nTx = (im[i].width + g.tw - 1) / g.tw;
+        if (r >= ntiles[i] / nTx) continue;
+        for (uint64_t c = 0; c < nTx; ++c, ++k) {
+          (*queue)[2 * (k * nx + x)] = i;
+          (*queue)[2 * (k * nx + x) + 1] = (uint32_t)(r * nTx + c);
+        }
+      }
+  }
   uint32_t base = 0;
   for (int i = 0; i < count; ++i) {
     BatchItem& it = (*items)[i];
@@ -461,20 +520,10 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
     it.g.nTy = nTy;
     it.g.vcarry = nullptr;  // rebased on the device address in sat_batch_prepare
     it.g.hcarry = nullptr;
-    it.g.vin = 0;
-    it.g.vout = 0;
     it.g.fault = nullptr;  // set on the device address in sat_batch_prepare
-    {
-      const size_t si = dtype_size(im[i].tin), so = dtype_size(im[i].tout);
-      const size_t ni = si * 4 >= 16 ? 16 : si * 4, no = so * 4 >= 16 ? 16 : so * 4;
-      it.g.vin = ((uintptr_t)im[i].in % ni == 0 && ((uint64_t)im[i].in_pitch * si) % ni == 0) ? 1u : 0u;
-      it.g.vout = ((uintptr_t)im[i].out % no == 0 && ((uint64_t)im[i].out_pitch * so) % no == 0) ? 1u : 0u;
-    }
     it.kind = (uint32_t)pair_id(im[i].tin, im[i].tout);
     it.tile_base = base;
-    const uint32_t n = nTx * nTy;
-    for (uint32_t k = 0; k < n; ++k) (*tile_img)[base + k] = (uint32_t)i;
-    base += n;
+    base += nTx * nTy;
   }
   return SAT_OK;
 }
@@ -682,8 +731,8 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
   if (!workspace) return SAT_ERR_NULL;
   BatchPlanInfo info;
   std::vector<BatchItem> items;
-  std::vector<uint32_t> tile_img;
-  sat_status s = batch_layout(images, count, -1, &info, &items, &tile_img);
+  std::vector<uint32_t> queue;
+  sat_status s = batch_layout(images, count, -1, &info, &items, &queue);
   if (s != SAT_OK) return s;
   if (workspace_bytes < info.total) return SAT_ERR_WORKSPACE;
   const Geometry g = batch_geometry(info.out_size, info.cfg);
@@ -699,8 +748,8 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
       hipMemcpyAsync((char*)workspace + info.items_off, items.data(), sizeof(BatchItem) * items.size(),
                      hipMemcpyHostToDevice, st) != hipSuccess)
     return SAT_ERR_HIP;
-  if (!tile_img.empty() &&
-      hipMemcpyAsync((char*)workspace + info.tile_img_off, tile_img.data(), sizeof(uint32_t) * tile_img.size(),
+  if (!queue.empty() &&
+      hipMemcpyAsync((char*)workspace + info.queue_off, queue.data(), sizeof(uint32_t) * queue.size(),
                      hipMemcpyHostToDevice, st) != hipSuccess)
     return SAT_ERR_HIP;
   if (hipStreamSynchronize(st) != hipSuccess) return SAT_ERR_HIP;
@@ -723,7 +772,7 @@ int sat_batch_run(void* workspace, void* stream) {
   const sat_status s = launch_bump_epoch(workspace, info.carry_words, st);
   if (s != SAT_OK) return s;
   return launch_batch(info.out_size, info.cfg, (const BatchItem*)((char*)workspace + info.items_off),
-                      (const uint32_t*)((char*)workspace + info.tile_img_off), info.total_tiles, workspace, st);
+                      (const uint32_t*)((char*)workspace + info.queue_off), info.blocks, workspace, st);
 }
 
 int sat_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height,

@@ -330,9 +330,13 @@ __device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t 
 }
 
 // ---------------------------------------------------------------------------
-// 4-wide global loads and stores of a row chunk [col, col+4) with zero fill
-// outside the image. `vec` selects one vector access when the chunk is whole
-// and the row base is aligned for it (decided on the host).
+// 4-wide global loads and stores of a row chunk [col, col+4).
+//
+// VecG<T> is 4 consecutive elements aligned to ONE element: code objects for
+// amdhsa run with unaligned global access enabled, and the compiler emits a
+// single dwordx4 (dword for 1-byte elements, two dwordx4 for 8-byte ones)
+// for these, so rows of any pitch take vector accesses. Vec4<T> (naturally
+// aligned) is kept for LDS.
 // ---------------------------------------------------------------------------
 template <typename T> struct Vec4;
 template <> struct Vec4<uint8_t> { typedef uint8_t type __attribute__((ext_vector_type(4))); };
@@ -342,12 +346,19 @@ template <> struct Vec4<int32_t> { typedef int32_t type __attribute__((ext_vecto
 template <> struct Vec4<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct Vec4<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
+template <typename T> struct VecG {
+  typedef T type __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+};
+
+// Whole chunks inside the image take one vector load; a chunk that crosses
+// the right edge or lies below the image is read element by element (zeros
+// outside).
 template <typename TSrc, typename TDst>
 __device__ __forceinline__ void load4(const TSrc* __restrict__ row, int64_t col, int64_t W, bool row_ok,
-                                      bool vec, arith_t<TDst> (&x)[4]) {
+                                      arith_t<TDst> (&x)[4]) {
   using A = arith_t<TDst>;
-  if (row_ok && vec && col + 3 < W) {
-    const typename Vec4<TSrc>::type q = *(const typename Vec4<TSrc>::type*)(row + col);
+  if (row_ok && col + 3 < W) {
+    const typename VecG<TSrc>::type q = *(const typename VecG<TSrc>::type*)(row + col);
     x[0] = cvt_in<TSrc, TDst>(q.x);
     x[1] = cvt_in<TSrc, TDst>(q.y);
     x[2] = cvt_in<TSrc, TDst>(q.z);
@@ -359,17 +370,41 @@ __device__ __forceinline__ void load4(const TSrc* __restrict__ row, int64_t col,
   }
 }
 
+// Edge tiles of images at least 4 wide: every lane issues one vector load
+// per row with no branch around it (so all rows' loads are in flight
+// together). `row` is the image row, already clamped into the image, and a
+// chunk that would cross the right edge reads the last whole chunk of the
+// row, [W-4, W), and shifts its elements into place; elements outside the
+// image (col + c >= W, or row_ok false) are zero.
+template <typename T>
+__device__ __forceinline__ T pick4(const T (&e)[4], int i) {
+  return i == 0 ? e[0] : i == 1 ? e[1] : i == 2 ? e[2] : e[3];
+}
+
+template <typename TSrc, typename TDst>
+__device__ __forceinline__ void load4_edge(const TSrc* __restrict__ row, int64_t col, int64_t W, bool row_ok,
+                                           arith_t<TDst> (&x)[4]) {
+  using A = arith_t<TDst>;
+  const int64_t base = col + 3 < W ? col : W - 4;
+  const typename VecG<TSrc>::type q = *(const typename VecG<TSrc>::type*)(row + base);
+  const TSrc e[4] = {q.x, q.y, q.z, q.w};
+  const int s = (int)(col - base);
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    x[c] = (row_ok && col + c < W) ? cvt_in<TSrc, TDst>(pick4(e, (c + s) & 3)) : zero<A>();
+}
+
 template <typename TDst>
-__device__ __forceinline__ void store4(TDst* __restrict__ row, int64_t col, int64_t W, bool row_ok, bool vec,
+__device__ __forceinline__ void store4(TDst* __restrict__ row, int64_t col, int64_t W, bool row_ok,
                                        const arith_t<TDst> (&y)[4]) {
   if (!row_ok) return;
-  if (vec && col + 3 < W) {
-    typename Vec4<TDst>::type q;
+  if (col + 3 < W) {
+    typename VecG<TDst>::type q;
     q.x = cvt_out<TDst>(y[0]);
     q.y = cvt_out<TDst>(y[1]);
     q.z = cvt_out<TDst>(y[2]);
     q.w = cvt_out<TDst>(y[3]);
-    *(typename Vec4<TDst>::type*)(row + col) = q;
+    *(typename VecG<TDst>::type*)(row + col) = q;
   } else {
 #pragma unroll
     for (int c = 0; c < 4; ++c)
@@ -416,7 +451,6 @@ struct ImgArgs {
   uint32_t nTx, nTy;  // tiles per tile-row / per tile-column
   uint64_t* vcarry;   // vertical carry words of this image's tile 0
   uint64_t* hcarry;   // horizontal carry words of this image's tile 0
-  uint32_t vin, vout; // whole-chunk vector loads / stores allowed
   uint32_t* fault;    // workspace fault word (look-back give-up)
 };
 

@@ -59,8 +59,9 @@ struct BatchItem {
   uint32_t tile_base;  // first global tile of this image
 };
 int pair_id(int tin, int tout);  // -1 if unsupported
-sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* tile_img,
-                        uint32_t total_tiles, void* ws, hipStream_t stream);
+// queue: (image, tile) pairs, one per workgroup (image 0xffffffff: empty)
+sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* queue, uint32_t blocks,
+                        void* ws, hipStream_t stream);
 Geometry batch_geometry(size_t out_size, int cfg);
 
 // Box filter (k_box.hip).

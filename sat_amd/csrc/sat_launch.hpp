@@ -39,12 +39,6 @@ inline Geometry geom_of() {
   return g;
 }
 
-inline bool aligned_for(const void* p, int64_t pitch, int64_t plane, size_t esize) {
-  const size_t need = esize * 4 >= 16 ? 16 : esize * 4;
-  const uintptr_t a = (uintptr_t)p;
-  return (a % need == 0) && (((uint64_t)pitch * esize) % need == 0) && (((uint64_t)plane * esize) % need == 0);
-}
-
 template <class Op>
 inline uint64_t op_carry_words(const Launch& L) {
   const uint64_t nTx = (L.width + Op::TW - 1) / Op::TW, nTy = (L.height + Op::TH - 1) / Op::TH;
@@ -76,8 +70,6 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   g.nTy = (uint32_t)nTy;
   g.vcarry = carries;
   g.hcarry = carries + vw;
-  g.vin = aligned_for(L.in, L.ldi, L.in_plane, sizeof(TSrc)) ? 1u : 0u;
-  g.vout = aligned_for(L.out, L.ldo, L.out_plane, sizeof(TDst)) ? 1u : 0u;
   g.fault = &((satdev::WsHeader*)ws)->fault;
   satdev::TileOrder order;
   order.nx = (uint32_t)device_xcd_count();

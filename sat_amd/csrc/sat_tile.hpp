@@ -128,19 +128,29 @@ struct TileOp {
   __device__ static __forceinline__ void run(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm) {
     const Coords k = coords(g, t);
     const TSrc* __restrict__ in = (const TSrc*)g.in;
-    const bool vin = g.vin != 0;
     // ---- load (each wave: RPT coalesced row segments of 256 elements) ----
     A v[RPT][4];
-    if (vin && interior(g, k)) {
-      typename Vec4<TSrc>::type q[RPT];
+    if (interior(g, k)) {
+      typename VecG<TSrc>::type q[RPT];
       const TSrc* p0 = in + prow_of(k, 0) * g.ldi + k.pcol;
       const int64_t step = REV ? -g.ldi : g.ldi;
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) q[r] = *(const typename Vec4<TSrc>::type*)(p0 + r * step);
+      for (int r = 0; r < RPT; ++r) q[r] = *(const typename VecG<TSrc>::type*)(p0 + r * step);
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const A x[4] = {cvt_in<TSrc, TDst>(q[r].x), cvt_in<TSrc, TDst>(q[r].y), cvt_in<TSrc, TDst>(q[r].z),
                         cvt_in<TSrc, TDst>(q[r].w)};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
+      }
+    } else if (g.W >= 4) {
+      // edge tile: clamped rows and chunks, one vector load per row
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int64_t prow = prow_of(k, r);
+        const bool ok = prow < (int64_t)g.H;
+        A x[4];
+        load4_edge<TSrc, TDst>(in + (ok ? prow : (int64_t)g.H - 1) * g.ldi, k.pcol, g.W, ok, x);
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
       }
@@ -149,7 +159,7 @@ struct TileOp {
       for (int r = 0; r < RPT; ++r) {
         const int64_t prow = prow_of(k, r);
         A x[4];
-        load4<TSrc, TDst>(in + prow * g.ldi, k.pcol, g.W, prow < (int64_t)g.H, vin, x);
+        load4<TSrc, TDst>(in + prow * g.ldi, k.pcol, g.W, prow < (int64_t)g.H, x);
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
       }
@@ -164,7 +174,6 @@ struct TileOp {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t I = k.I, J = k.J;
     const int64_t py0 = k.py0, px0 = k.px0, pcol = k.pcol;
-    const bool vout = g.vout != 0;
 
     // ---- per-wave column sums; the tile's column sums are published first ----
     A colsum[CPV];
@@ -349,7 +358,7 @@ struct TileOp {
     // ---- store ----
     if constexpr (!TOUT) {
       TDst* __restrict__ out = (TDst*)g.out;
-      const bool fast = vout && interior(g, k);
+      const bool fast = interior(g, k);
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int li = w * RPT + r;
@@ -363,7 +372,7 @@ struct TileOp {
           y[REV ? 3 - c : c] = s;
         }
         if (fast) {
-          typename Vec4<TDst>::type q;
+          typename VecG<TDst>::type q;
           q.x = cvt_out<TDst>(y[0]);
           q.y = cvt_out<TDst>(y[1]);
           q.z = cvt_out<TDst>(y[2]);
@@ -371,9 +380,9 @@ struct TileOp {
           TDst* o0 = out + (py0 + (REV ? TH - 1 - w * RPT : w * RPT)) * g.ldo + pcol;
           // streaming output: nontemporal stores keep it from displacing
           // the carry words and the input in the caches
-          __builtin_nontemporal_store(q, (typename Vec4<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo)));
+          __builtin_nontemporal_store(q, (typename VecG<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo)));
         } else {
-          store4<TDst>(out + prow * g.ldo, pcol, g.W, prow < (int64_t)g.H, vout, y);
+          store4<TDst>(out + prow * g.ldo, pcol, g.W, prow < (int64_t)g.H, y);
         }
       }
     } else {
@@ -446,7 +455,7 @@ struct SerialTileOp {
       for (int r = r0; r < TH; r += RPL) {
         const int64_t row = y0 + r;
         A x[4];
-        load4<TSrc, TDst>(in + row * g.ldi, col, g.W, row < (int64_t)g.H, g.vin != 0, x);
+        load4<TSrc, TDst>(in + row * g.ldi, col, g.W, row < (int64_t)g.H, x);
         stage_put4<A, TW>(sm.st, r, 4 * seg, x);
       }
     }
