@@ -66,7 +66,10 @@ struct TileOp {
   static constexpr int HWAVE = NW - 1;
   // corner look-back: its own wave when there is one to spare
   static constexpr int CWAVE = (NW >= 3 && NVW <= NW - 2) ? NW - 2 : HWAVE;
-  static constexpr int64_t VS = TW + (CORNER ? 1 : 0);  // vertical entries per tile
+  // vertical entries per tile: the tile's columns, then its corner entry padded
+  // to a whole 64-byte line, so every tile's block of column words starts on a
+  // line and its wave-wide stores write whole lines
+  static constexpr int64_t VS = TW + (CORNER ? 8 : 0);
   static constexpr int64_t VWORDS = DO_COL ? VS * Carry<A>::WORDS : 0;  // per tile
   static constexpr int64_t HWORDS = DO_ROW ? (int64_t)TH * Carry<A>::WORDS : 0;  // per tile
   static_assert(!(TOUT && (REV || DO_COL)), "transposed store is a row-scan pass only");

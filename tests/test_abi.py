@@ -100,12 +100,14 @@ def test_batch_validation(L):
 
 
 def test_workspace_sizes(L):
-    # one-pass f32: 64x256 tiles, 256 column + 1 corner + 64 row carry words each
+    # one-pass f32: the largest over the tile configurations is the 64x256 tile:
+    # 256 column words + the corner word padded to a 64-byte line (8 words)
+    # + 64 row words each
     tiles = (8192 // 64) * (8192 // 256)
-    assert L.sat_integral_workspace_bytes(3, 6, 6, 8192, 8192) == 256 + tiles * (257 + 64) * 8
+    assert L.sat_integral_workspace_bytes(3, 6, 6, 8192, 8192) == 256 + tiles * (264 + 64) * 8
     # f64: 32x256 tiles, two words per carry
     tiles = (4096 // 32) * (4096 // 256)
-    assert L.sat_integral_workspace_bytes(3, 7, 7, 4096, 4096) == 256 + tiles * (257 + 32) * 16
+    assert L.sat_integral_workspace_bytes(3, 7, 7, 4096, 4096) == 256 + tiles * (264 + 32) * 16
     # two-pass algorithms add a temporary of the output type
     assert L.sat_integral_workspace_bytes(2, 6, 6, 1000, 1000) >= 1024 * 1000 * 4
     assert L.sat_integral_workspace_bytes(0, 0, 4, 1000, 1000) >= 1024 * 1000 * 4
