@@ -60,8 +60,54 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(objs)} translation units", file=sys.stderr)
+    build_cli(verbose)
     return LIB
+
+
+CLI_SRC = PKG / "cli" / "sat_cli.cpp"
+CLI = PKG / "bin" / "sat"
+
+
+def build_cli(verbose: bool = True) -> Path:
+    """The `sat` command-line driver (host code over the C ABI), linked against
+    libsat_amd.so with an rpath to it."""
+    CLI.parent.mkdir(exist_ok=True)
+    tmp = CLI.with_suffix(".tmp")
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-Wno-unused-command-line-argument", str(CLI_SRC),
+           f"-L{PKG}",
+           "-lsat_amd", "-Wl,-rpath,$ORIGIN/..", "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, CLI)
+    if verbose:
+        print(f"built {CLI}", file=sys.stderr)
+    return CLI
+
+
+TESTS_CPP = PKG.parent / "tests" / "cpp"
+
+
+def build_cpp_tests(verbose: bool = True) -> list:
+    """C++ programs of the test suite (tests/cpp/*.cpp, host code over the C
+    ABI and the C++ headers) -> tests/cpp/_build/<name>."""
+    out = []
+    if not TESTS_CPP.is_dir():
+        return out
+    (TESTS_CPP / "_build").mkdir(exist_ok=True)
+    for src in sorted(TESTS_CPP.glob("*.cpp")):
+        exe = TESTS_CPP / "_build" / src.stem
+        cmd = [_hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-Wno-unused-command-line-argument", str(src),
+               f"-L{PKG}", "-lsat_amd", "-Wl,-rpath,$ORIGIN/../../../sat_amd", "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"test program {src.name} failed to build:\n{r.stderr[-4000:]}")
+        out.append(exe)
+    if verbose and out:
+        print(f"built {len(out)} C++ test program(s) in {TESTS_CPP / '_build'}", file=sys.stderr)
+    return out
 
 
 if __name__ == "__main__":
     build()
+    build_cpp_tests()
