@@ -146,6 +146,20 @@ int sat_integral_planes(int alg, int scan, int tin, int tout, const void* in, in
                         uint32_t width, uint32_t height, uint32_t planes, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* OpenCV-style layout (cv::integral): `out` is (height+1) x (width+1) with
+ * row 0 and column 0 zero and out[y+1][x+1] = sum_{k<=y, l<=x} in[k][l];
+ * out_pitch >= width + 1 elements. Same algorithms, pairs and workspace as
+ * sat_integral (sat_integral_workspace_bytes of width x height). The
+ * reference's README compares against OpenCV's integral (README.md:29); no
+ * reference entry point has this layout. */
+int sat_integral_bordered(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out,
+                          int64_t out_pitch, uint32_t width, uint32_t height, void* workspace, size_t workspace_bytes,
+                          void* stream);
+int sat_integral_bordered_planes(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch,
+                                 int64_t in_plane_stride, void* out, int64_t out_pitch, int64_t out_plane_stride,
+                                 uint32_t width, uint32_t height, uint32_t planes, void* workspace,
+                                 size_t workspace_bytes, void* stream);
+
 /* Reverse-direction SAT (the adjoint of sat_integral):
  * out[y][x] = sum_{k>=y, l>=x} in[k][l]. One pass. tin == tout, f32 or f64
  * (also u32 / i32, which wrap). No reference counterpart. */

@@ -62,6 +62,9 @@ SIGNATURES = {
     "sat_box_scatter": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp]),
     "sat_box_grad_workspace_bytes": (_sz, [_i, _u32, _u32, _u32, _u32]),
     "sat_box_grad": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp, _sz, _vp]),
+    "sat_integral_bordered": (_i, [_i, _i, _i, _i, _vp, _i64, _vp, _i64, _u32, _u32, _vp, _sz, _vp]),
+    "sat_integral_bordered_planes": (_i, [_i, _i, _i, _i, _vp, _i64, _i64, _vp, _i64, _i64, _u32, _u32, _u32, _vp,
+                                          _sz, _vp]),
     "sat_tile_config_count": (_i, []),
     "sat_tune_class": (None, [_u32, _u32, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sat_tune_get": (_i, [_i, _i, _i, _i]),

@@ -1,5 +1,7 @@
 // One-pass SAT kernels (forward and reverse) in tile configuration 0, the
 // configuration dispatch, the fill / epoch kernels and the family geometry table.
+#include <algorithm>
+
 #include "sat_launch.hpp"
 
 namespace satdev {
@@ -25,9 +27,36 @@ __global__ void __launch_bounds__(256) bump_epoch_kernel(WsHeader* hdr, uint64_t
   if (threadIdx.x == 0) __hip_atomic_store(&hdr->epoch, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Zero border of an OpenCV-style (H+1) x (W+1) table: row 0 (W+1 elements)
+// and column 0 of rows 1..H, `planes` tables `plane` elements apart.
+template <typename T>
+__global__ void __launch_bounds__(256) border_kernel(T* out, int64_t pitch, int64_t plane, uint32_t width,
+                                                     uint32_t height) {
+  T* o = out + (int64_t)blockIdx.y * plane;
+  const uint64_t n = (uint64_t)width + 1 + height;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i <= width) o[i] = T(0);
+    else o[(int64_t)(i - width) * pitch] = T(0);
+  }
+}
+
 }  // namespace satdev
 
 namespace sathost {
+
+sat_status launch_border(size_t out_size, void* out, int64_t pitch, int64_t plane, uint32_t width, uint32_t height,
+                         uint32_t planes, hipStream_t stream) {
+  const uint64_t n = (uint64_t)width + 1 + height;
+  uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 1024);
+  if (planes == 0) return SAT_OK;
+  if (out_size == 8)
+    hipLaunchKernelGGL(satdev::border_kernel<uint64_t>, dim3(blocks, planes), dim3(256), 0, stream, (uint64_t*)out, pitch,
+                       plane, width, height);
+  else
+    hipLaunchKernelGGL(satdev::border_kernel<uint32_t>, dim3(blocks, planes), dim3(256), 0, stream, (uint32_t*)out, pitch,
+                       plane, width, height);
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
 
 using namespace satdev;
 

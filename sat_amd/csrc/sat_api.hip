@@ -680,6 +680,35 @@ int sat_integral(int alg, int scan, int tin, int tout, const void* in, int64_t i
                              workspace_bytes, stream);
 }
 
+int sat_integral_bordered_planes(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch,
+                                 int64_t in_plane_stride, void* out, int64_t out_pitch, int64_t out_plane_stride,
+                                 uint32_t width, uint32_t height, uint32_t planes, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  if (!scan_ok(scan) || alg < 0 || alg > SAT_ALG_SINGLE_PASS) return SAT_ERR_INVALID;
+  if (!pair_ok(tin, tout)) return SAT_ERR_DTYPE;
+  if (planes == 0) return SAT_OK;
+  if (!out) return SAT_ERR_NULL;
+  if (out_pitch < (int64_t)width + 1) return SAT_ERR_INVALID;
+  if (planes > 1 && out_plane_stride < out_pitch * ((int64_t)height + 1)) return SAT_ERR_INVALID;
+  const size_t osz = dtype_size(tout);
+  hipStream_t st = (hipStream_t)stream;
+  // the table proper starts one row and one column in
+  if (width > 0 && height > 0) {
+    const int s = sat_integral_planes(alg, scan, tin, tout, in, in_pitch, in_plane_stride,
+                                      (char*)out + (size_t)(out_pitch + 1) * osz, out_pitch, out_plane_stride, width,
+                                      height, planes, workspace, workspace_bytes, stream);
+    if (s != SAT_OK) return s;
+  }
+  return launch_border(osz, out, out_pitch, out_plane_stride, width, height, planes, st);
+}
+
+int sat_integral_bordered(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out,
+                          int64_t out_pitch, uint32_t width, uint32_t height, void* workspace, size_t workspace_bytes,
+                          void* stream) {
+  return sat_integral_bordered_planes(alg, scan, tin, tout, in, in_pitch, 0, out, out_pitch, 0, width, height, 1,
+                                      workspace, workspace_bytes, stream);
+}
+
 size_t sat_integral_reverse_planes_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes) {
   Launch L = make_launch(FAM_SAT_REV, 0, dtype, dtype, nullptr, width, nullptr, width, width, height);
   L.planes = planes;

@@ -15,7 +15,7 @@ from . import (BRLT_SCANROW, F32, F64, I8, I32, KOGGE_STONE, SCANROW_BRLT, SCANR
                SatError, check, lib, sat_image)
 
 __all__ = [
-    "sat_dtype", "default_out_dtype", "integral", "integral_reverse", "block_scan", "seriel_scan", "scan_x", "scan_y",
+    "sat_dtype", "default_out_dtype", "integral", "integral_bordered", "integral_reverse", "block_scan", "seriel_scan", "scan_x", "scan_y",
     "box_sums", "box_scatter", "box_grad", "BatchPlan", "sat", "box_sum", "box_filter", "SATFunction",
     "BoxSumFunction", "BoxFilterFunction", "SCANROW_BRLT", "BRLT_SCANROW", "SCANROW_COLUMN", "SINGLE_PASS",
 ]
@@ -81,6 +81,25 @@ def integral(x: torch.Tensor, out_dtype: torch.dtype | None = None, alg: int = S
                                    x3.stride(0), o3.data_ptr(), o3.stride(1), o3.stride(0), w, h, p, None, 0,
                                    _stream(x))
     check(st, "sat_integral_planes")
+    return out
+
+
+def integral_bordered(x: torch.Tensor, out_dtype: torch.dtype | None = None, alg: int = SINGLE_PASS,
+                      scan: int = KOGGE_STONE) -> torch.Tensor:
+    """OpenCV ``cv::integral`` layout: (..., H+1, W+1) with a zero first row and
+    column; out[..., y+1, x+1] is the inclusive 2-D prefix sum at (y, x)."""
+    _dev_check(x)
+    out_dtype = out_dtype or default_out_dtype(x.dtype)
+    x3 = _planes(x)
+    p, h, w = x3.shape
+    out = torch.empty((*x.shape[:-2], h + 1, w + 1), dtype=out_dtype, device=x.device)
+    o3 = out.view(-1, h + 1, w + 1)
+    if p == 0:
+        return out
+    st = lib().sat_integral_bordered_planes(alg, scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x3.data_ptr(),
+                                            x3.stride(1), x3.stride(0), o3.data_ptr(), o3.stride(1), o3.stride(0), w,
+                                            h, p, None, 0, _stream(x))
+    check(st, "sat_integral_bordered_planes")
     return out
 
 

@@ -144,3 +144,37 @@ def test_planes(O, dev, sa):
     for n in range(2):
         for c in range(3):
             assert_sat_close(O, x[n, c], y[n, c], "f32")
+
+
+@pytest.mark.parametrize("tin,tout", [("u8", "u32"), ("i8", "i32"), ("f32", "f32"), ("f64", "f64")])
+def test_integral_bordered(O, dev, sa, tin, tout):
+    """OpenCV cv::integral layout: (H+1) x (W+1), zero first row and column."""
+    say(f"[case] bordered {tin}->{tout}")
+    for k, (h, w) in enumerate([(1, 1), (3, 5), (130, 520), (257, 2049)]):
+        for alg in ((0, 1, 2, 3) if k == 2 else (3,)):
+            x = O.gen(KIND[tin], (h, w), 3000 + 7 * k + alg)
+            y = to_np(sa.integral_bordered(to_dev(x, dev), out_dtype=TD[tout], alg=alg))
+            assert y.shape == (h + 1, w + 1)
+            assert not y[0].any() and not y[:, 0].any()
+            assert_sat_close(O, x, np.ascontiguousarray(y[1:, 1:]), tout)
+
+
+def test_integral_bordered_pitched_planes(O, dev, sa):
+    """Row pitch beyond W+1 and several planes; the border is written even
+    over garbage, elements past W+1 in each row are left alone."""
+    import ctypes as C
+
+    say("[case] bordered pitched planes")
+    P, h, w, pitch = 3, 70, 300, 320
+    x = O.gen("u8", (P, h, w), 55)
+    xd = to_dev(x, dev)
+    out = torch.full((P, h + 1, pitch), -1, dtype=torch.int32, device=dev)
+    st = sa.lib().sat_integral_bordered_planes(3, 0, 0, 4, xd.data_ptr(), w, h * w, out.data_ptr(), pitch,
+                                               (h + 1) * pitch, w, h, P, None, 0,
+                                               torch.cuda.current_stream().cuda_stream)
+    assert st == 0
+    o = out.cpu().numpy().view(np.uint32)
+    for p in range(P):
+        assert not o[p, 0, :w + 1].any() and not o[p, :, 0].any()
+        assert np.array_equal(o[p, 1:, 1:w + 1], O.serial(x[p], np.uint32))
+        assert (o[p, :, w + 1:] == 0xFFFFFFFF).all()
