@@ -38,3 +38,15 @@ def test_cpp_dropin_program():
     say("[case] tests/cpp dropin_test")
     r = subprocess.run([str(DROPIN)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "dropin ok" in r.stdout, r.stdout[-3000:] + r.stderr[-2000:]
+
+
+def test_sweep_harness(tmp_path):
+    say("[case] sweep harness, 2 algorithms x 2 pairs x 2 sizes")
+    from sat_amd import sweep
+
+    rows = sweep.sweep(algs=(0, 3), pairs=[(0, 4), (6, 6)], n=2, csv=tmp_path / "s.csv", check=True,
+                       log=lambda *m: say(*m))
+    assert rows[0] == sweep.HEADER and len(rows) == 1 + 2 * 2 * 2
+    for r in rows[1:]:
+        f = r.split(",")
+        assert int(f[4]) in (1024, 2048) and float(f[6]) > 0 and float(f[7]) > 0
