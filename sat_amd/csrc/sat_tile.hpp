@@ -521,13 +521,18 @@ struct SerialTileOp {
 // ---------------------------------------------------------------------------
 struct TileOrder {
   uint32_t nx;   // XCDs the workgroups are dealt over
-  uint32_t run;  // tile columns per XCD and tile row (0: identity order)
+  uint32_t run;  // tile columns of one run (0: identity order); divides nTx / nx
 };
 
+// Runs of `run` tile columns are dealt to the XCDs round-robin (run q of a
+// tile row to XCD q % nx); each XCD walks its runs row by row, left to right.
 __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nTx, TileOrder o) {
   if (o.run == 0) return b;
   const uint32_t x = b % o.nx, k = b / o.nx;
-  return (k / o.run) * nTx + x * o.run + (k % o.run);
+  const uint32_t per = nTx / o.nx;  // tiles of one tile row on one XCD
+  const uint32_t I = k / per, j = k - I * per;
+  const uint32_t q = (j / o.run) * o.nx + x;
+  return I * nTx + q * o.run + (j % o.run);
 }
 
 // Four waves per SIMD (at most 128 registers per lane): the tile kernels are
