@@ -20,9 +20,10 @@
 //   horizontal words per tile row: the tile's row sums (AGG), the row sums up
 //                    to and including the tile (INC).
 //
-// Tiles are numbered by tickets taken from a counter when a workgroup starts,
-// row-major over the tile grid, so every predecessor a tile waits for has
-// already been handed to a running workgroup.
+// Workgroups map to tiles through tile_of_block (below): every tile's upper
+// and left predecessors belong to workgroups with smaller indices, which the
+// hardware starts earlier, so a tile only ever waits for tiles that are
+// resident or finished.
 #pragma once
 
 #include "sat_device.hpp"
