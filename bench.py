@@ -140,6 +140,20 @@ def main():
     algo_bytes = H * W * (4 + 4)
     gbs = algo_bytes / (ms * 1e-3) / 1e9
 
+    # measured ceiling: the library's streaming copy of the same bytes (the
+    # SAT's 16-byte loads and nontemporal stores without any carries)
+    log("[bench] timing the streaming-copy ceiling on the same bytes")
+    y2 = torch.empty_like(x)
+    lib = sat_amd.lib()
+    cstream = torch.cuda.current_stream().cuda_stream
+
+    def copy_step():
+        sat_amd.check(lib.sat_stream_copy(x.data_ptr(), y2.data_ptr(), H * W * 4, cstream), "sat_stream_copy")
+
+    cms = time_launches(copy_step, args.steps, args.warmup)
+    copy_gbs = algo_bytes / (cms * 1e-3) / 1e9
+    del y2
+
     extra = {}
     if not args.no_extra:
         extra = extra_configs(O, sat_amd, dev)
@@ -182,7 +196,10 @@ def main():
                    "f32_max_rel_err_vs_fp64": rel},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": algo_bytes},
+                     "algorithmic_bytes": algo_bytes,
+                     "copy_ceiling": {"kernel": "sat_stream_copy, 256 MiB in + 256 MiB out",
+                                      "GBs": round(copy_gbs, 1), "ms": round(cms, 5),
+                                      "sat_frac_of_copy": round(gbs / copy_gbs, 4)}},
         "cpu_baseline": cpu,
     }
     if extra:

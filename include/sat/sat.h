@@ -85,6 +85,12 @@ size_t sat_dtype_size(int dtype);
  * carry does not arrive within about a second gives up and raises the fault. */
 int sat_workspace_fault(void* workspace, void* stream);
 
+/* Streaming device-to-device copy of `bytes` bytes (16-byte aligned,
+ * non-overlapping buffers), stream-ordered. No counterpart in the reference:
+ * it is the measured bandwidth ceiling the SAT kernels are compared against
+ * (the same 16-byte vector loads and nontemporal stores, no carries). */
+int sat_stream_copy(const void* src, void* dst, size_t bytes, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Pass-level drop-ins. Pitches (`widthStride`, `heightStride`) are in elements
  * of the pointed-to type, as GpuData::DataPitch() returns them

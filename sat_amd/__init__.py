@@ -41,6 +41,7 @@ SIGNATURES = {
     "sat_dtype_pair_supported": (_i, [_i, _i]),
     "sat_status_string": (C.c_char_p, [_i]),
     "sat_workspace_fault": (_i, [_vp, _vp]),
+    "sat_stream_copy": (_i, [_vp, _vp, _sz, _vp]),
     "sat_block_scan": (_i, [_i, _i, _i, _vp, _vp, _u32, _u32, _u32, _u32, _vp]),
     "sat_seriel_scan": (_i, [_i, _i, _vp, _vp, _u32, _u32, _u32, _u32, _vp]),
     "sat_scan_x": (_i, [_i, _i, _i, _vp, _vp, _u32, _u32, _u32, _vp]),

@@ -11,6 +11,31 @@ __global__ void __launch_bounds__(256) clear_kernel(uint64_t* p, uint64_t n) {
     p[i] = 0;
 }
 
+// Streaming copy, the bandwidth reference the SAT kernels are measured
+// against: each thread moves COPY_UNROLL 16-byte chunks, every load of a wave
+// covering 1 KiB of consecutive bytes, all loads issued before the first
+// (nontemporal) store. Chunks past `n16` are skipped; `tail` bytes after the
+// last whole chunk are copied by the first workgroup.
+constexpr int COPY_UNROLL = 4;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) stream_copy_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                          uint64_t n16, const uint8_t* __restrict__ src_tail,
+                                                          uint8_t* __restrict__ dst_tail, uint32_t tail) {
+  const uint64_t base = (uint64_t)blockIdx.x * (256 * COPY_UNROLL) + threadIdx.x;
+  u32x4 v[COPY_UNROLL];
+#pragma unroll
+  for (int u = 0; u < COPY_UNROLL; ++u) {
+    const uint64_t i = base + (uint64_t)u * 256;
+    if (i < n16) v[u] = src[i];
+  }
+#pragma unroll
+  for (int u = 0; u < COPY_UNROLL; ++u) {
+    const uint64_t i = base + (uint64_t)u * 256;
+    if (i < n16) __builtin_nontemporal_store(v[u], dst + i);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < tail) dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
 // Advances the device epoch of a workspace for the launch that follows on the
 // same stream; on wrap-around it clears `words` carry words first.
 __global__ void __launch_bounds__(256) bump_epoch_kernel(WsHeader* hdr, uint64_t* carries, uint64_t words) {
@@ -63,6 +88,17 @@ using namespace satdev;
 sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream) {
   hipLaunchKernelGGL(bump_epoch_kernel, dim3(1), dim3(256), 0, stream, (WsHeader*)ws,
                      (uint64_t*)((char*)ws + WS_HEADER_BYTES), carry_words);
+  return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
+}
+
+sat_status launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t stream) {
+  const uint64_t n16 = bytes / 16;
+  const uint32_t tail = (uint32_t)(bytes % 16);
+  const uint64_t per_block = 256 * COPY_UNROLL;
+  const uint64_t blocks = std::max<uint64_t>(1, (n16 + per_block - 1) / per_block);
+  if (blocks > 0x7fffffffu) return SAT_ERR_INVALID;
+  hipLaunchKernelGGL(stream_copy_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, (const u32x4*)src,
+                     (u32x4*)dst, n16, (const uint8_t*)src + n16 * 16, (uint8_t*)dst + n16 * 16, tail);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 

@@ -611,6 +611,15 @@ int sat_workspace_fault(void* workspace, void* stream) {
   return v ? 1 : 0;
 }
 
+int sat_stream_copy(const void* src, void* dst, size_t bytes, void* stream) {
+  if (bytes == 0) return SAT_OK;
+  if (!src || !dst) return SAT_ERR_NULL;
+  if (((uintptr_t)src | (uintptr_t)dst) % 16 != 0) return SAT_ERR_INVALID;
+  const uintptr_t s0 = (uintptr_t)src, d0 = (uintptr_t)dst;
+  if (s0 < d0 + bytes && d0 < s0 + bytes) return SAT_ERR_INVALID;  // overlapping buffers
+  return launch_stream_copy(src, dst, bytes, (hipStream_t)stream);
+}
+
 int sat_block_scan(int scan, int tin, int tout, const void* in, void* out, uint32_t width, uint32_t widthStride,
                    uint32_t height, uint32_t heightStride, void* stream) {
   return pass_call(FAM_ROWS_T, scan, tin, tout, in, widthStride, out, heightStride, width, height, stream);

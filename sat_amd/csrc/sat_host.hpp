@@ -80,6 +80,7 @@ size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t
 
 // Zero `bytes` (rounded down to 8) at `p` with a fill kernel (k_sat.hip).
 sat_status launch_clear(void* p, size_t bytes, hipStream_t stream);
+sat_status launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 // Zero row 0 and column 0 of `planes` (H+1) x (W+1) tables (k_sat.hip).
 sat_status launch_border(size_t out_size, void* out, int64_t pitch, int64_t plane, uint32_t width, uint32_t height,
                          uint32_t planes, hipStream_t stream);

@@ -80,6 +80,11 @@ def test_validation_without_device(L):
     assert L.sat_batch_run(p, None) == 2  # never prepared
     # planes with overlapping plane strides
     assert L.sat_integral_planes(3, 0, 6, 6, p, 64, 10, p, 64, 64 * 64, 64, 64, 2, None, 0, None) == 2
+    # streaming copy: empty is a no-op; null, misaligned and overlapping buffers are refused
+    assert L.sat_stream_copy(None, None, 0, None) == 0
+    assert L.sat_stream_copy(None, p, 16, None) == 1
+    assert L.sat_stream_copy(C.c_void_p(0x1008), C.c_void_p(0x9000), 16, None) == 2
+    assert L.sat_stream_copy(p, C.c_void_p(0x1010), 64, None) == 2
 
 
 def test_batch_validation(L):

@@ -178,3 +178,17 @@ def test_integral_bordered_pitched_planes(O, dev, sa):
         assert not o[p, 0, :w + 1].any() and not o[p, :, 0].any()
         assert np.array_equal(o[p, 1:, 1:w + 1], O.serial(x[p], np.uint32))
         assert (o[p, :, w + 1:] == 0xFFFFFFFF).all()
+
+
+@pytest.mark.parametrize("nbytes", [1, 15, 16, 17, 4096, 1_000_003, (64 << 20) + 7])
+def test_stream_copy(dev, sa, nbytes):
+    """The bandwidth-reference copy moves every byte, tails included."""
+    say(f"[stream_copy] {nbytes} bytes")
+    g = torch.Generator(device="cpu").manual_seed(nbytes)
+    src = torch.randint(0, 256, (nbytes,), dtype=torch.uint8, generator=g).to(dev)
+    dst = torch.zeros(nbytes + 64, dtype=torch.uint8, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    sa.check(sa.lib().sat_stream_copy(src.data_ptr(), dst.data_ptr(), nbytes, st), "sat_stream_copy")
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:nbytes], src)
+    assert int(dst[nbytes:].count_nonzero()) == 0
