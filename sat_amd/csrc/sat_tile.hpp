@@ -20,10 +20,12 @@
 //   horizontal words per tile row: the tile's row sums (AGG), the row sums up
 //                    to and including the tile (INC).
 //
-// Workgroups map to tiles through tile_of_block (below): every tile's upper
-// and left predecessors belong to workgroups with smaller indices, which the
-// hardware starts earlier, so a tile only ever waits for tiles that are
-// resident or finished.
+// Workgroups map to tiles through tile_of_block (below), under which the
+// workgroups each XCD starts (in increasing index) take tiles in row-major
+// order. The first unfinished tile in row-major order therefore has every
+// earlier tile of its XCD finished, so it is resident, and its predecessors
+// (above and to the left, earlier in row-major order) are finished: the
+// launch always progresses.
 #pragma once
 
 #include "sat_device.hpp"
