@@ -363,7 +363,9 @@ def diff_config(O, sat_amd, dev, reps=10):
 
 
 def extra_configs(O, sat_amd, dev):
-    """Secondary workloads (each median-timed): the other algorithms, dtypes and shapes."""
+    """Secondary workloads: the other algorithms, dtypes and shapes. Each is the
+    average of back-to-back calls (as the headline), except the 2048^2 lines:
+    those are median-timed one call at a time behind a 512 MiB write."""
     import numpy as np
     import torch
 
@@ -376,9 +378,13 @@ def extra_configs(O, sat_amd, dev):
         f = lambda: sat_amd.integral(x, out_dtype=out_dtype, alg=alg, out=y)  # noqa: E731
         f()
         torch.cuda.synchronize()
-        # small images fit in the 256 MiB MALL: a 512 MiB write before every
-        # timed call keeps them from being served from it
-        ms = time_each(f, steps, (lambda: flush_buf.fill_(1)) if flush else None)
+        if flush:
+            # small images fit in the 256 MiB MALL: a 512 MiB write before every
+            # timed call keeps them from being served from it (and keeps the GPU
+            # busy while the call is submitted, so no launch gap is timed)
+            ms = time_each(f, steps, lambda: flush_buf.fill_(1))
+        else:
+            ms = time_launches(f, steps, 2)
         gbs = x.numel() * (in_b + out_b) / (ms * 1e-3) / 1e9
         res[name] = {"ms": round(ms, 4), "GBs": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 1)}
         log(f"[bench] {name}: {ms:.4f} ms {gbs:.1f} GB/s")
