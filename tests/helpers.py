@@ -39,3 +39,33 @@ def assert_sat_close(O, x: np.ndarray, got: np.ndarray, tout: str, exact: np.nda
     ratio = float(np.max(err / (u * scale + 1e-300))) if err.size else 0.0
     assert np.all(err <= bound), f"max error {ratio:.2f} u*SAT(|x|) exceeds {k}"
     return ratio
+
+
+def assert_inverts(S, x, chunk: int = 2048) -> None:
+    """Checks that the 2-D difference S(i,j) - S(i-1,j) - S(i,j-1) + S(i-1,j-1)
+    (S = 0 outside) gives back the input x modulo 2^32 at every pixel, which
+    holds exactly when S is the inclusive 2-D prefix sum of x with 32-bit
+    wraparound (the difference map is a bijection on 32-bit images). S and x
+    are (H, W) torch tensors on any one device: S of a 32-bit integer dtype, x
+    of any integer dtype; the work is done `chunk` rows at a time in int64."""
+    import torch
+
+    M32 = 0xFFFFFFFF
+    H, W = S.shape
+    Si = S.view(torch.int32)
+    if x.dtype == torch.uint32:
+        x = x.view(torch.int32)
+    prev = torch.zeros((1, W), dtype=torch.int64, device=S.device)
+    for r0 in range(0, H, chunk):
+        blk = Si[r0:r0 + chunk].to(torch.int64) & M32
+        d = blk - torch.cat([prev, blk[:-1]], 0)
+        d2 = d.clone()
+        d2[:, 1:] -= d[:, :-1]
+        d2 &= M32
+        xe = x[r0:r0 + chunk].to(torch.int64) & M32
+        if not torch.equal(d2, xe):
+            bad = (d2 != xe).nonzero()[0].tolist()
+            i, j = bad[0] + r0, bad[1]
+            raise AssertionError(f"2-D difference differs from the input at ({i}, {j}): "
+                                 f"{int(d2[bad[0], j])} != {int(xe[bad[0], j])}")
+        prev = blk[-1:]

@@ -153,3 +153,29 @@ def test_box_sums_brute_force(O):
         assert got[p, b] == pytest.approx(float(np.sum(x[p, y0:y1 + 1, x0:x1 + 1])), rel=1e-12, abs=1e-12)
     bad = np.array([[[0, 0, 20, 3]]], np.int32)
     assert np.isnan(O.box_sums_f64(sat[:1], bad)[0, 0])
+
+
+@pytest.mark.parametrize("kind,tin,tout", [("u8", np.uint8, np.uint32), ("i32_16", np.int32, np.int32),
+                                           ("u32", np.uint32, np.uint32)])
+def test_difference_checker(O, kind, tin, tout):
+    """The full-size GPU tests check integer SATs by their 2-D difference
+    (helpers.assert_inverts). It must accept the oracle's SAT, wraps included,
+    and reject a SAT with one planted error or with a row dropped from the fold."""
+    import torch
+
+    from helpers import assert_inverts
+
+    def t(a):
+        return torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a)
+
+    x = O.gen(kind, (300, 170), 99)
+    S = O.serial(x, tout)
+    assert_inverts(t(S), t(x), chunk=64)
+    bad = S.copy()
+    bad[123, 45] += 1
+    with pytest.raises(AssertionError):
+        assert_inverts(t(bad), t(x), chunk=64)
+    shifted = np.zeros_like(S)
+    shifted[1:] = S[:-1]  # the SAT of the image without its last row, shifted down
+    with pytest.raises(AssertionError):
+        assert_inverts(t(shifted), t(x), chunk=64)
