@@ -55,7 +55,10 @@ def _planes(x: torch.Tensor):
     if x.dim() < 2:
         raise SatError("expected a tensor of at least 2 dimensions")
     h, w = x.shape[-2], x.shape[-1]
-    x3 = x.reshape(-1, h, w)
+    p = 1
+    for d in x.shape[:-2]:
+        p *= d
+    x3 = x.reshape(p, h, w)  # explicit plane count: also valid for empty images
     if x3.stride(-1) != 1 or (x3.shape[0] > 1 and x3.stride(0) < x3.stride(1) * h):
         x3 = x3.contiguous()
     return x3
@@ -74,7 +77,7 @@ def integral(x: torch.Tensor, out_dtype: torch.dtype | None = None, alg: int = S
         out = torch.empty(x.shape, dtype=out_dtype, device=x.device)
     if out.dtype != out_dtype or out.shape != x.shape or not out.is_contiguous():
         raise SatError("`out` must be a contiguous tensor of the input's shape and the output dtype")
-    o3 = out.view(-1, h, w)
+    o3 = out.view(p, h, w)
     if x3.numel() == 0:
         return out
     st = lib().sat_integral_planes(alg, scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x3.data_ptr(), x3.stride(1),
@@ -93,7 +96,7 @@ def integral_bordered(x: torch.Tensor, out_dtype: torch.dtype | None = None, alg
     x3 = _planes(x)
     p, h, w = x3.shape
     out = torch.empty((*x.shape[:-2], h + 1, w + 1), dtype=out_dtype, device=x.device)
-    o3 = out.view(-1, h + 1, w + 1)
+    o3 = out.view(p, h + 1, w + 1)
     if p == 0:
         return out
     st = lib().sat_integral_bordered_planes(alg, scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x3.data_ptr(),
@@ -111,7 +114,7 @@ def integral_reverse(x: torch.Tensor, scan: int = KOGGE_STONE) -> torch.Tensor:
     out = torch.empty(x.shape, dtype=x.dtype, device=x.device)
     if x3.numel() == 0:
         return out
-    o3 = out.view(-1, h, w)
+    o3 = out.view(p, h, w)
     st = lib().sat_integral_reverse_planes(scan, sat_dtype(x.dtype), x3.data_ptr(), x3.stride(1), x3.stride(0),
                                            o3.data_ptr(), o3.stride(1), o3.stride(0), w, h, p, None, 0, _stream(x))
     check(st, "sat_integral_reverse_planes")

@@ -192,3 +192,18 @@ def test_stream_copy(dev, sa, nbytes):
     torch.cuda.synchronize()
     assert torch.equal(dst[:nbytes], src)
     assert int(dst[nbytes:].count_nonzero()) == 0
+
+
+def test_empty_inputs(dev, sa):
+    """Empty images are no-ops with outputs of the right shape; the bordered
+    layout of an empty image is its zero border alone."""
+    say("[case] empty inputs")
+    for shape in ((0, 5), (5, 0), (0, 0), (2, 0, 3), (0, 4, 4)):
+        x = torch.zeros(shape, dtype=torch.float32, device=dev)
+        y = sa.integral(x)
+        assert y.shape == x.shape and y.dtype == torch.float32
+        assert sa.integral_reverse(x).shape == x.shape
+        b = sa.integral_bordered(x.to(torch.uint8), out_dtype=torch.uint32)
+        assert b.shape == (*shape[:-2], shape[-2] + 1, shape[-1] + 1)
+        if b.numel():
+            assert int(b.view(torch.int32).count_nonzero()) == 0
