@@ -125,8 +125,13 @@ def main():
         y2 = torch.empty_like(x)
         for _ in range(3):
             y2.copy_(x)
+        lib = sat_amd.lib()
+        for _ in range(args.steps):
+            sat_amd.check(lib.sat_stream_copy(x.data_ptr(), y2.data_ptr(), H * W * 4,
+                                              torch.cuda.current_stream().cuda_stream), "sat_stream_copy")
         torch.cuda.synchronize()
-        log(f"[bench] profile run done: {args.steps} SAT launches, 3 copies of {H * W * 4} bytes")
+        log(f"[bench] profile run done: {args.steps} SAT launches, 3 torch copies and {args.steps} "
+            f"sat_stream_copy calls of {H * W * 4} bytes")
         return
     # parity of the measured output (not timed)
     log("[bench] checking the output against the FP64 oracle")
