@@ -11,7 +11,7 @@ run goes under ``rocprofv3 --kernel-trace --stats -f csv``, which writes the
 per-kernel trace CSV the reference's nvprof runs produced.
 
     python -m sat_amd.sweep --n 8 --csv sweep.csv
-    python -m sat_amd.sweep --algs 3 --pairs 6:6 --n 16 --rocprof gpurun_out/sweep_prof
+    python -m sat_amd.sweep --algs 3 --pairs 6:6 --n 16 --rocprof sweep_prof
 """
 from __future__ import annotations
 

@@ -27,8 +27,9 @@ CFG_NAMES = {0: "4 waves, 64x256 (32x256 for 8-byte outputs)",
              1: "8 waves, 128x256 (64x256)",
              2: "8 waves, 64x256 (32x256), half the registers per lane"}
 
-# (height, width, planes) shapes that stand for their class; the BASELINE
-# workloads plus square, tall, wide and plane-stacked sweeps
+# (height, width, planes) shapes that stand for their class; the benchmark
+# workloads (f32 8192^2, u8->u32 16384^2, f64 4096^2, 65536x256, 256x65536,
+# 96 planes of 1024^2, ...) plus square, tall, wide and plane-stacked sweeps
 FULL_SHAPES = [
     (2048, 2048, 1), (4096, 4096, 1), (8192, 8192, 1), (16384, 16384, 1),
     (1024, 1024, 1), (512, 512, 1), (256, 256, 1),
