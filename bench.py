@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: the headline SAT --steps times, then 3 plain copies of the same "
                          "bytes (the counter calibration), no checks and no CPU baseline")
+    ap.add_argument("--profile-alg", type=int, default=3, choices=[0, 1, 2, 3],
+                    help="algorithm of the --profile run (0-2: the reference's two-pass compositions)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the SAT path here has no multi-GPU mode "
@@ -113,8 +115,10 @@ def main():
     x = torch.from_numpy(x_np).to(dev)
     y = torch.empty_like(x)
 
+    alg = args.profile_alg if args.profile else sat_amd.SINGLE_PASS
+
     def step():
-        sat_amd.integral(x, alg=sat_amd.SINGLE_PASS, out=y)
+        sat_amd.integral(x, alg=alg, out=y)
 
     step()
     torch.cuda.synchronize()
