@@ -68,18 +68,22 @@ def cpu_baseline(x, tout, budget_s: float = 12.0):
     for the OpenMP path on all host cores and the serial path on one core."""
     from oracle import oracle as O
 
+    import statistics
+
     res = {}
     ncores = os.cpu_count() or 1
     O.set_threads(ncores)
+    res["omp_threads"] = O.threads()
     for name, fn, threads in (("parallel", O.parallel, ncores), ("serial", O.serial, 1)):
         fn(x, tout)  # warm-up
-        reps = 6
-        t0 = time.perf_counter()
-        for _ in range(reps):
+        ts = []
+        for _ in range(6):
+            t0 = time.perf_counter()
             fn(x, tout)
-        dt = (time.perf_counter() - t0) / reps
-        res[name] = (dt, threads)
-        if dt * reps > budget_s:
+            ts.append(time.perf_counter() - t0)
+        res[name] = (sum(ts) / len(ts), threads)
+        res[name + "_median"] = statistics.median(ts)
+        if sum(ts) > budget_s:
             break
     return res
 
@@ -173,10 +177,13 @@ def main():
         r = cpu_baseline(x_np, np.float32)
         dt, cores = r["parallel"]
         cpu = {"value": algo_bytes / dt / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
-               "sample": f"IntegralImageParallel f32 {H}x{W}, OpenMP on all host cores, mean of 6 runs"}
+               "sample": f"IntegralImageParallel f32 {H}x{W}, OpenMP on all host cores, mean of 6 runs",
+               "median_GBs": algo_bytes / r["parallel_median"] / 1e9, "nproc": os.cpu_count(),
+               "omp_threads": r["omp_threads"]}
         if "serial" in r:
             dts, _ = r["serial"]
             cpu["serial_1core_GBs"] = algo_bytes / dts / 1e9
+            cpu["serial_1core_median_GBs"] = algo_bytes / r["serial_median"] / 1e9
 
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this same workload, scaled by a plain copy of known size (the committed
