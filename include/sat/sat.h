@@ -219,8 +219,10 @@ int sat_batch_run(void* workspace, void* stream);
  *   class_x = min(floor(log2(ceil(width / 256))), 8)
  *   class_y = min(floor(log2(ceil(height / 64))), 11)
  *   class_p = 0, 1, 2, 3 for 1, 2-3, 4-15, 16+ planes.
- * The table starts with built-in choices; sat_amd.tune measures every
- * configuration per class on the device and fills it in. Results of integer
+ * The table starts as the tuned table shipped with the build (measured by
+ * sat_amd.tune on the target device and compiled in); classes it does not
+ * list, and classes reset with cfg = -1, use the built-in rule (narrow images
+ * of >= 4096 rows take configuration 1, the rest 0). Results of integer
  * pairs are identical in every configuration. Workspace sizes returned by the
  * *_workspace_bytes functions fit every configuration.
  * ------------------------------------------------------------------------- */

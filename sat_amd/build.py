@@ -46,8 +46,41 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+TUNED_JSON = PKG / "tuned_gfx950.json"
+TUNED_INC = CSRC / "tuned_table.inc"
+
+
+def write_tuned_table() -> Path:
+    """The autotuner's table (TUNED_JSON, written by ``python -m sat_amd.tune``)
+    as C++ initialisers compiled into the library, so C and C++ callers start
+    from the tuned tile configurations too. A table for another architecture
+    than the build's is left out."""
+    import json
+
+    entries, batch, arch = [], [], None
+    if TUNED_JSON.exists():
+        doc = json.loads(TUNED_JSON.read_text())
+        arch = doc.get("arch")
+        if arch == ARCH:
+            entries = [(int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]), int(e["class_p"]), int(e["cfg"]))
+                       for e in doc.get("entries", [])]
+            batch = [(int(e["out_bytes"]), int(e["cfg"])) for e in doc.get("batch", [])]
+    lines = [f"// Generated by sat_amd/build.py from sat_amd/{TUNED_JSON.name} (arch {arch}, build {ARCH}).",
+             "// {out_bytes, class_x, class_y, class_p, cfg}",
+             "static const ShippedTune k_shipped_tune[] = {"]
+    lines += [f"    {{{ob}, {cx}, {cy}, {cp}, {cfg}}}," for ob, cx, cy, cp, cfg in entries]
+    lines += ["    {0, 0, 0, 0, -1},  // end", "};", "// {out_bytes, cfg}", "static const ShippedBatch k_shipped_batch[] = {"]
+    lines += [f"    {{{ob}, {cfg}}}," for ob, cfg in batch]
+    lines += ["    {0, -1},  // end", "};", ""]
+    text = "\n".join(lines)
+    if not TUNED_INC.exists() or TUNED_INC.read_text() != text:
+        TUNED_INC.write_text(text)
+    return TUNED_INC
+
+
 def build(verbose: bool = True, jobs: int | None = None) -> Path:
     OUT_DIR.mkdir(exist_ok=True)
+    write_tuned_table()
     srcs = sorted(CSRC.glob("*.hip"))
     jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:

@@ -67,6 +67,30 @@ constexpr int TUNE_CX = 9, TUNE_CY = 12, TUNE_CP = 4;
 static std::atomic<int> g_tune[2][TUNE_CX][TUNE_CY][TUNE_CP];  // cfg + 1
 static std::atomic<int> g_tune_batch[2];                        // cfg + 1
 
+// The autotuner's shipped table (sat_amd/tuned_gfx950.json, turned into
+// tuned_table.inc by the build) is the table's content when the library is
+// loaded; sat_tune_set overrides an entry, -1 returns it to the built-in rule.
+struct ShippedTune {
+  int out_bytes, cx, cy, cp, cfg;
+};
+struct ShippedBatch {
+  int out_bytes, cfg;
+};
+#include "tuned_table.inc"
+
+static bool load_shipped_table() {
+  for (const ShippedTune& e : k_shipped_tune) {
+    if (e.cfg < 0 || e.cfg >= TILE_CFGS || (e.out_bytes != 4 && e.out_bytes != 8)) continue;
+    if (e.cx < 0 || e.cx >= TUNE_CX || e.cy < 0 || e.cy >= TUNE_CY || e.cp < 0 || e.cp >= TUNE_CP) continue;
+    g_tune[e.out_bytes == 8 ? 1 : 0][e.cx][e.cy][e.cp].store(e.cfg + 1, std::memory_order_relaxed);
+  }
+  for (const ShippedBatch& e : k_shipped_batch)
+    if (e.cfg >= 0 && e.cfg < TILE_CFGS && (e.out_bytes == 4 || e.out_bytes == 8))
+      g_tune_batch[e.out_bytes == 8 ? 1 : 0].store(e.cfg + 1, std::memory_order_relaxed);
+  return true;
+}
+static const bool g_shipped_loaded = load_shipped_table();
+
 static int floor_log2(uint64_t v) {
   int r = 0;
   while (v > 1) {

@@ -58,6 +58,37 @@ def test_tuned_table_loaded():
     assert r.returncode == 0 and "table ok" in r.stdout, r.stderr[-2000:]
 
 
+def test_library_starts_from_shipped_table():
+    """The library itself (loaded without the Python package) starts from the
+    shipped table compiled into it, so C and C++ callers get the tuned tiles."""
+    import subprocess
+    import sys
+
+    code = (
+        "import ctypes as C, json, pathlib\n"
+        "root = pathlib.Path('.').resolve()\n"
+        "L = C.CDLL(str(root / 'sat_amd' / 'libsat_amd.so'))\n"
+        "doc = json.loads((root / 'sat_amd' / 'tuned_gfx950.json').read_text())\n"
+        "for e in doc['entries']:\n"
+        "    assert L.sat_tune_get(e['out_bytes'], e['class_x'], e['class_y'], e['class_p']) == e['cfg'], e\n"
+        "for e in doc['batch']:\n"
+        "    assert L.sat_batch_tune_get(e['out_bytes']) == e['cfg'], e\n"
+        "assert L.sat_tune_set(4, 5, 7, 0, -1) == 0 and L.sat_tune_get(4, 5, 7, 0) == 0\n"
+        "print('compiled table ok')\n")
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "compiled table ok" in r.stdout, r.stderr[-2000:]
+
+
+def test_tuned_table_include_is_current():
+    """sat_amd/csrc/tuned_table.inc is what the build generates from the JSON."""
+    from sat_amd import build as b
+
+    before = b.TUNED_INC.read_text()
+    b.write_tuned_table()
+    assert b.TUNED_INC.read_text() == before
+
+
 def test_builtin_choice(L):
     from sat_amd.tune import reset_table
 
