@@ -207,3 +207,24 @@ def test_empty_inputs(dev, sa):
         assert b.shape == (*shape[:-2], shape[-2] + 1, shape[-1] + 1)
         if b.numel():
             assert int(b.view(torch.int32).count_nonzero()) == 0
+
+
+def test_concurrent_streams(O, dev, sa):
+    """Calls on different streams run concurrently, each on its own
+    library-managed carry workspace (re-entrant across streams)."""
+    say("[case] concurrent streams")
+    xs = [O.gen("f32", (2048, 3000), 500 + k) for k in range(4)]
+    streams = [torch.cuda.Stream() for _ in xs]
+    outs = []
+    for x, s in zip(xs, streams):
+        xd = torch.from_numpy(x).to(dev)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            for _ in range(3):  # several launches per stream, interleaved with the others
+                y = sa.integral(xd)
+        outs.append((x, y, xd))
+    torch.cuda.synchronize()
+    for x, y, _ in outs:
+        assert_sat_close(O, x, to_np(y), "f32")
+    for s in streams:
+        assert sa.lib().sat_workspace_fault(None, s.cuda_stream) == 0
