@@ -89,7 +89,7 @@ static bool load_shipped_table() {
       g_tune_batch[e.out_bytes == 8 ? 1 : 0].store(e.cfg + 1, std::memory_order_relaxed);
   return true;
 }
-static const bool g_shipped_loaded = load_shipped_table();
+[[maybe_unused]] static const bool g_shipped_loaded = load_shipped_table();
 
 static int floor_log2(uint64_t v) {
   int r = 0;

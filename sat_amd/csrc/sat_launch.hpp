@@ -48,8 +48,6 @@ inline uint64_t op_carry_words(const Launch& L) {
 
 template <class Op>
 inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStream_t stream) {
-  using TSrc = typename Op::Src;
-  using TDst = typename Op::Dst;
   const uint64_t nTx = (L.width + Op::TW - 1) / Op::TW, nTy = (L.height + Op::TH - 1) / Op::TH;
   const uint64_t tpp = nTx * nTy, ntiles = tpp * L.planes;
   if (ntiles == 0) return SAT_OK;
