@@ -14,7 +14,7 @@ struct FamRows {
 
 template <typename S, typename D, int SC>
 struct FamRowsT {
-  using Op = TileOp<S, D, MODE_ROW, false, true, SC, kNW, kRPT<D>>;
+  using Op = TileOp<S, D, MODE_ROW, false, true, SC, kNW_T, kRPT_T<D>>;
   static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
 };
 

@@ -141,8 +141,8 @@ Geometry geometry(int family, size_t out_size, int cfg) {
       return b8 ? geom_of<TileOp<double, double, MODE_ROW, false, false, 0, kNW, kRPT<double>>>()
                 : geom_of<TileOp<float, float, MODE_ROW, false, false, 0, kNW, kRPT<float>>>();
     case FAM_ROWS_T:
-      return b8 ? geom_of<TileOp<double, double, MODE_ROW, false, true, 0, kNW, kRPT<double>>>()
-                : geom_of<TileOp<float, float, MODE_ROW, false, true, 0, kNW, kRPT<float>>>();
+      return b8 ? geom_of<TileOp<double, double, MODE_ROW, false, true, 0, kNW_T, kRPT_T<double>>>()
+                : geom_of<TileOp<float, float, MODE_ROW, false, true, 0, kNW_T, kRPT_T<float>>>();
     case FAM_COLS:
       return b8 ? geom_of<TileOp<double, double, MODE_COL, false, false, 0, kNW, kRPT<double>>>()
                 : geom_of<TileOp<float, float, MODE_COL, false, false, 0, kNW, kRPT<float>>>();

@@ -13,6 +13,11 @@ using satdev::arith_t;
 template <typename TDst> constexpr int kRPT = sizeof(arith_t<TDst>) == 4 ? 16 : 8;
 template <typename TDst> constexpr int kSerialCW = sizeof(arith_t<TDst>) == 4 ? 64 : 32;
 constexpr int kNW = 4;
+// The transposed row pass (blockScan) stages its whole tile in LDS for the
+// transpose, so LDS and not registers bounds its workgroups per CU: it uses
+// 8 waves of half the rows (tiles of the same height, four waves per SIMD).
+constexpr int kNW_T = 8;
+template <typename TDst> constexpr int kRPT_T = kRPT<TDst> / 2;
 
 // Tile configurations of the one-pass SAT (chosen per launch by the tuning
 // table):

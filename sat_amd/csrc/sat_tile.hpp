@@ -393,8 +393,10 @@ struct TileOp {
       }
     } else {
       // Block-register-local transpose: stage the scanned tile in LDS, then
-      // write output rows (= input columns); the TH lanes of a group write TH
-      // consecutive output elements.
+      // write output rows (= input columns). Each lane gathers 4 consecutive
+      // tile rows of one column and writes them as one 4-element vector, so
+      // TH/4 lanes cover an output row's TH elements and every store
+      // instruction of a wave writes 256/TH whole row segments.
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         A y[4];
@@ -403,17 +405,33 @@ struct TileOp {
         stage_put4<A, TW>(sm.st, w * RPT + r, 4 * lane, y);
       }
       __syncthreads();
-      constexpr int GROUPS = NT / TH;  // output rows written per pass
-      static_assert(NT % TH == 0 && TW % GROUPS == 0, "transpose passes");
+      constexpr int LPR = TH / 4;    // lanes per output row segment
+      constexpr int RPI = NT / LPR;  // output rows written per pass
+      static_assert(TH % 4 == 0 && NT % LPR == 0 && TW % RPI == 0, "transpose passes");
       TDst* __restrict__ out = (TDst*)g.out;
-      const int yl = tid % TH, grp = tid / TH;
-      const int64_t oy = py0 + yl;
-      const bool y_ok = oy < (int64_t)g.H;
-#pragma unroll 8
-      for (int xo = grp; xo < TW; xo += GROUPS) {
+      const int seg = tid % LPR, rsub = tid / LPR;
+      const int64_t oy0 = py0 + 4 * seg;
+      const bool full = oy0 + 4 <= (int64_t)g.H;
+#pragma unroll 4
+      for (int xo = rsub; xo < TW; xo += RPI) {
         const int64_t orow = px0 + xo;
-        const A val = stage_get<A, TW>(sm.st, yl, xo);
-        if (y_ok && orow < (int64_t)g.W) out[orow * g.ldo + oy] = cvt_out<TDst>(val);
+        A y[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) y[c] = stage_get<A, TW>(sm.st, 4 * seg + c, xo);
+        if (orow < (int64_t)g.W) {
+          if (full) {
+            typename VecG<TDst>::type q;
+            q.x = cvt_out<TDst>(y[0]);
+            q.y = cvt_out<TDst>(y[1]);
+            q.z = cvt_out<TDst>(y[2]);
+            q.w = cvt_out<TDst>(y[3]);
+            __builtin_nontemporal_store(q, (typename VecG<TDst>::type*)(out + orow * g.ldo + oy0));
+          } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+              if (oy0 + c < (int64_t)g.H) out[orow * g.ldo + oy0 + c] = cvt_out<TDst>(y[c]);
+          }
+        }
       }
     }
   }
