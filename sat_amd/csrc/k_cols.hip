@@ -16,7 +16,7 @@ struct FamCols {
 // map to the same kernel.
 template <typename S, typename D, int SC>
 struct FamSerialT {
-  using Op = SerialTileOp<S, D, kNW, kSerialCW<D>>;
+  using Op = SerialTileOp<S, D, kNW_S, kSerialCW<D>>;
   static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
 };
 

@@ -147,8 +147,8 @@ Geometry geometry(int family, size_t out_size, int cfg) {
       return b8 ? geom_of<TileOp<double, double, MODE_COL, false, false, 0, kNW, kRPT<double>>>()
                 : geom_of<TileOp<float, float, MODE_COL, false, false, 0, kNW, kRPT<float>>>();
     case FAM_SERIAL_T:
-      return b8 ? geom_of<SerialTileOp<double, double, kNW, kSerialCW<double>>>()
-                : geom_of<SerialTileOp<float, float, kNW, kSerialCW<float>>>();
+      return b8 ? geom_of<SerialTileOp<double, double, kNW_S, kSerialCW<double>>>()
+                : geom_of<SerialTileOp<float, float, kNW_S, kSerialCW<float>>>();
     default: {
       Geometry g{0, 0, 0, 0, 0};
       return g;

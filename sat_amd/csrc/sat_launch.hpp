@@ -11,7 +11,11 @@ using satdev::arith_t;
 // Default tile shapes: 4 waves; 16 rows per wave for 4-byte outputs (64x256
 // tiles, 64 accumulators per lane), 8 rows per wave for 8-byte outputs.
 template <typename TDst> constexpr int kRPT = sizeof(arith_t<TDst>) == 4 ? 16 : 8;
-template <typename TDst> constexpr int kSerialCW = sizeof(arith_t<TDst>) == 4 ? 64 : 32;
+// serielScan pass: 8 waves of kSerialCW columns each (tiles of 64 rows x 256
+// columns, 64 x 128 for 8-byte outputs); its LDS stage bounds it to two
+// workgroups per CU, so 8 waves keep four per SIMD.
+template <typename TDst> constexpr int kSerialCW = sizeof(arith_t<TDst>) == 4 ? 32 : 16;
+constexpr int kNW_S = 8;
 constexpr int kNW = 4;
 // The transposed row pass (blockScan) stages its whole tile in LDS for the
 // transpose, so LDS and not registers bounds its workgroups per CU: it uses

@@ -442,8 +442,9 @@ struct TileOp {
 // row scan). The tile (64 rows x NW*CW columns) is loaded row-major, moved
 // through LDS so that lane = tile row and registers = a run of CW consecutive
 // columns owned by the wave, scanned serially in registers, joined across the
-// waves' column runs and across tiles (horizontal look-back), and stored so
-// that for every register the 64 lanes write 64 consecutive output elements.
+// waves' column runs and across tiles (horizontal look-back), and moved back
+// through LDS so that every lane stores 4 consecutive output elements (4 tile
+// rows of one column) as one vector.
 // ---------------------------------------------------------------------------
 template <typename TSrc, typename TDst, int NW, int CW>
 struct SerialTileOp {
@@ -471,17 +472,41 @@ struct SerialTileOp {
     const int64_t y0 = (int64_t)I * TH, x0 = (int64_t)J * TW;
     const TSrc* __restrict__ in = (const TSrc*)g.in;
 
-    // ---- coalesced row-major load into LDS ----
+    // ---- coalesced row-major load into LDS (all of a lane's loads are
+    //      issued before the first is staged) ----
     {
+      constexpr int NLOAD = TH / RPL;
       const int seg = tid % LPR, r0 = tid / LPR;
       const int64_t col = x0 + 4 * seg;
-#pragma unroll 4
-      for (int r = r0; r < TH; r += RPL) {
-        const int64_t row = y0 + r;
-        A x[4];
-        load4<TSrc, TDst>(in + row * g.ldi, col, g.W, row < (int64_t)g.H, x);
-        stage_put4<A, TW>(sm.st, r, 4 * seg, x);
+      A x[NLOAD][4];
+      if (y0 + TH <= (int64_t)g.H && x0 + TW <= (int64_t)g.W) {
+        // interior tile: straight-line vector loads
+#pragma unroll
+        for (int i = 0; i < NLOAD; ++i) {
+          const typename VecG<TSrc>::type q =
+              *(const typename VecG<TSrc>::type*)(in + (y0 + r0 + i * RPL) * g.ldi + col);
+          x[i][0] = cvt_in<TSrc, TDst>(q.x);
+          x[i][1] = cvt_in<TSrc, TDst>(q.y);
+          x[i][2] = cvt_in<TSrc, TDst>(q.z);
+          x[i][3] = cvt_in<TSrc, TDst>(q.w);
+        }
+      } else if (g.W >= 4) {
+        // edge tile: clamped rows and chunks, one vector load per row
+#pragma unroll
+        for (int i = 0; i < NLOAD; ++i) {
+          const int64_t row = y0 + r0 + i * RPL;
+          const bool ok = row < (int64_t)g.H;
+          load4_edge<TSrc, TDst>(in + (ok ? row : (int64_t)g.H - 1) * g.ldi, col, g.W, ok, x[i]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NLOAD; ++i) {
+          const int64_t row = y0 + r0 + i * RPL;
+          load4<TSrc, TDst>(in + row * g.ldi, col, g.W, row < (int64_t)g.H, x[i]);
+        }
       }
+#pragma unroll
+      for (int i = 0; i < NLOAD; ++i) stage_put4<A, TW>(sm.st, r0 + i * RPL, 4 * seg, x[i]);
     }
     __syncthreads();
 
@@ -514,14 +539,43 @@ struct SerialTileOp {
     __syncthreads();
     base += sm.rp[lane];
 
-    // ---- transposed store: out[x][y], 64 lanes = 64 consecutive y ----
-    TDst* __restrict__ out = (TDst*)g.out;
-    const int64_t y = y0 + lane;
-    const bool y_ok = y < (int64_t)g.H;
+    // ---- transposed store: the results go back through LDS so that every
+    //      lane writes 4 consecutive y of one output row x as one vector ----
 #pragma unroll
-    for (int k = 0; k < CW; ++k) {
-      const int64_t x = x0 + w * CW + k;
-      if (y_ok && x < (int64_t)g.W) out[x * g.ldo + y] = cvt_out<TDst>(v[k] + base);
+    for (int m = 0; m < CW / 4; ++m) {
+      A y4[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) y4[c] = v[4 * m + c] + base;
+      stage_put4<A, TW>(sm.st, lane, w * CW + 4 * m, y4);
+    }
+    __syncthreads();
+    constexpr int SLPR = TH / 4;    // lanes per output row segment
+    constexpr int SRPI = NT / SLPR; // output rows written per pass
+    static_assert(NT % SLPR == 0 && TW % SRPI == 0, "transposed store passes");
+    TDst* __restrict__ out = (TDst*)g.out;
+    const int seg = tid % SLPR, rsub = tid / SLPR;
+    const int64_t oy0 = y0 + 4 * seg;
+    const bool full = oy0 + 4 <= (int64_t)g.H;
+#pragma unroll 4
+    for (int xo = rsub; xo < TW; xo += SRPI) {
+      const int64_t x = x0 + xo;
+      A q4[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) q4[c] = stage_get<A, TW>(sm.st, 4 * seg + c, xo);
+      if (x < (int64_t)g.W) {
+        if (full) {
+          typename VecG<TDst>::type q;
+          q.x = cvt_out<TDst>(q4[0]);
+          q.y = cvt_out<TDst>(q4[1]);
+          q.z = cvt_out<TDst>(q4[2]);
+          q.w = cvt_out<TDst>(q4[3]);
+          __builtin_nontemporal_store(q, (typename VecG<TDst>::type*)(out + x * g.ldo + oy0));
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            if (oy0 + c < (int64_t)g.H) out[x * g.ldo + oy0 + c] = cvt_out<TDst>(q4[c]);
+        }
+      }
     }
   }
 };
@@ -534,9 +588,10 @@ struct SerialTileOp {
 // neighbours then sit on its own XCD, started just before it, and the row
 // chains cross XCDs only between runs (in XCD order, so the dies settle into
 // a staggered pipeline instead of waiting on each other in a cycle). The map
-// keeps every XCD's sequence increasing in tile index, so a tile still only
-// waits for tiles that are resident or finished; the bounded look-back turns
-// a violation of that order into a reported fault instead of a hang.
+// keeps every XCD's sequence in row-major tile order, which guarantees
+// progress (see the top of this file); the bounded look-back turns a
+// violation of the hardware's dispatch order into a reported fault instead of
+// a hang.
 // order.run == 0 keeps tile = blockIdx.x. Tile t covers plane t / tpp, tile
 // t % tpp of that plane (planes stack vertically in the tile grid).
 // ---------------------------------------------------------------------------
