@@ -350,6 +350,13 @@ template <typename T> struct VecG {
   typedef T type __attribute__((ext_vector_type(4), aligned(sizeof(T))));
 };
 
+// Transposed stores write one 16-byte vector per lane (4 elements of 4 bytes
+// or 2 of 8 bytes), so the lanes of a row segment cover it without gaps.
+template <typename T> constexpr int kVecOut = 16 / (int)sizeof(T);
+template <typename T> struct VecOut {
+  typedef T type __attribute__((ext_vector_type(kVecOut<T>), aligned(sizeof(T))));
+};
+
 // Whole chunks inside the image take one vector load; a chunk that crosses
 // the right edge or lies below the image is read element by element (zeros
 // outside).

@@ -393,10 +393,9 @@ struct TileOp {
       }
     } else {
       // Block-register-local transpose: stage the scanned tile in LDS, then
-      // write output rows (= input columns). Each lane gathers 4 consecutive
-      // tile rows of one column and writes them as one 4-element vector, so
-      // TH/4 lanes cover an output row's TH elements and every store
-      // instruction of a wave writes 256/TH whole row segments.
+      // write output rows (= input columns). Each lane gathers E consecutive
+      // tile rows of one column and writes them as one 16-byte vector, so
+      // TH/E lanes cover an output row's TH elements without gaps.
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         A y[4];
@@ -405,30 +404,29 @@ struct TileOp {
         stage_put4<A, TW>(sm.st, w * RPT + r, 4 * lane, y);
       }
       __syncthreads();
-      constexpr int LPR = TH / 4;    // lanes per output row segment
-      constexpr int RPI = NT / LPR;  // output rows written per pass
-      static_assert(TH % 4 == 0 && NT % LPR == 0 && TW % RPI == 0, "transpose passes");
+      constexpr int E = kVecOut<TDst>;  // elements per lane store
+      constexpr int LPR = TH / E;       // lanes per output row segment
+      constexpr int RPI = NT / LPR;     // output rows written per pass
+      static_assert(TH % E == 0 && NT % LPR == 0 && TW % RPI == 0, "transpose passes");
       TDst* __restrict__ out = (TDst*)g.out;
       const int seg = tid % LPR, rsub = tid / LPR;
-      const int64_t oy0 = py0 + 4 * seg;
-      const bool full = oy0 + 4 <= (int64_t)g.H;
+      const int64_t oy0 = py0 + E * seg;
+      const bool full = oy0 + E <= (int64_t)g.H;
 #pragma unroll 4
       for (int xo = rsub; xo < TW; xo += RPI) {
         const int64_t orow = px0 + xo;
-        A y[4];
+        A y[E];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) y[c] = stage_get<A, TW>(sm.st, 4 * seg + c, xo);
+        for (int c = 0; c < E; ++c) y[c] = stage_get<A, TW>(sm.st, E * seg + c, xo);
         if (orow < (int64_t)g.W) {
           if (full) {
-            typename VecG<TDst>::type q;
-            q.x = cvt_out<TDst>(y[0]);
-            q.y = cvt_out<TDst>(y[1]);
-            q.z = cvt_out<TDst>(y[2]);
-            q.w = cvt_out<TDst>(y[3]);
-            __builtin_nontemporal_store(q, (typename VecG<TDst>::type*)(out + orow * g.ldo + oy0));
+            typename VecOut<TDst>::type q;
+#pragma unroll
+            for (int c = 0; c < E; ++c) q[c] = cvt_out<TDst>(y[c]);
+            __builtin_nontemporal_store(q, (typename VecOut<TDst>::type*)(out + orow * g.ldo + oy0));
           } else {
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < E; ++c)
               if (oy0 + c < (int64_t)g.H) out[orow * g.ldo + oy0 + c] = cvt_out<TDst>(y[c]);
           }
         }
@@ -443,8 +441,8 @@ struct TileOp {
 // through LDS so that lane = tile row and registers = a run of CW consecutive
 // columns owned by the wave, scanned serially in registers, joined across the
 // waves' column runs and across tiles (horizontal look-back), and moved back
-// through LDS so that every lane stores 4 consecutive output elements (4 tile
-// rows of one column) as one vector.
+// through LDS so that every lane stores consecutive output elements (tile
+// rows of one column) as one 16-byte vector.
 // ---------------------------------------------------------------------------
 template <typename TSrc, typename TDst, int NW, int CW>
 struct SerialTileOp {
@@ -540,7 +538,8 @@ struct SerialTileOp {
     base += sm.rp[lane];
 
     // ---- transposed store: the results go back through LDS so that every
-    //      lane writes 4 consecutive y of one output row x as one vector ----
+    //      lane writes E consecutive y of one output row x as one 16-byte
+    //      vector ----
 #pragma unroll
     for (int m = 0; m < CW / 4; ++m) {
       A y4[4];
@@ -549,31 +548,30 @@ struct SerialTileOp {
       stage_put4<A, TW>(sm.st, lane, w * CW + 4 * m, y4);
     }
     __syncthreads();
-    constexpr int SLPR = TH / 4;    // lanes per output row segment
-    constexpr int SRPI = NT / SLPR; // output rows written per pass
-    static_assert(NT % SLPR == 0 && TW % SRPI == 0, "transposed store passes");
+    constexpr int E = kVecOut<TDst>;  // elements per lane store
+    constexpr int SLPR = TH / E;      // lanes per output row segment
+    constexpr int SRPI = NT / SLPR;   // output rows written per pass
+    static_assert(TH % E == 0 && NT % SLPR == 0 && TW % SRPI == 0, "transposed store passes");
     TDst* __restrict__ out = (TDst*)g.out;
     const int seg = tid % SLPR, rsub = tid / SLPR;
-    const int64_t oy0 = y0 + 4 * seg;
-    const bool full = oy0 + 4 <= (int64_t)g.H;
+    const int64_t oy0 = y0 + E * seg;
+    const bool full = oy0 + E <= (int64_t)g.H;
 #pragma unroll 4
     for (int xo = rsub; xo < TW; xo += SRPI) {
       const int64_t x = x0 + xo;
-      A q4[4];
+      A qv[E];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) q4[c] = stage_get<A, TW>(sm.st, 4 * seg + c, xo);
+      for (int c = 0; c < E; ++c) qv[c] = stage_get<A, TW>(sm.st, E * seg + c, xo);
       if (x < (int64_t)g.W) {
         if (full) {
-          typename VecG<TDst>::type q;
-          q.x = cvt_out<TDst>(q4[0]);
-          q.y = cvt_out<TDst>(q4[1]);
-          q.z = cvt_out<TDst>(q4[2]);
-          q.w = cvt_out<TDst>(q4[3]);
-          __builtin_nontemporal_store(q, (typename VecG<TDst>::type*)(out + x * g.ldo + oy0));
+          typename VecOut<TDst>::type q;
+#pragma unroll
+          for (int c = 0; c < E; ++c) q[c] = cvt_out<TDst>(qv[c]);
+          __builtin_nontemporal_store(q, (typename VecOut<TDst>::type*)(out + x * g.ldo + oy0));
         } else {
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
-            if (oy0 + c < (int64_t)g.H) out[x * g.ldo + oy0 + c] = cvt_out<TDst>(q4[c]);
+          for (int c = 0; c < E; ++c)
+            if (oy0 + c < (int64_t)g.H) out[x * g.ldo + oy0 + c] = cvt_out<TDst>(qv[c]);
         }
       }
     }
