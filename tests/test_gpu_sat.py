@@ -228,3 +228,30 @@ def test_concurrent_streams(O, dev, sa):
         assert_sat_close(O, x, to_np(y), "f32")
     for s in streams:
         assert sa.lib().sat_workspace_fault(None, s.cuda_stream) == 0
+
+
+def test_random_shapes_and_pitches(O, dev, sa):
+    """A seeded sweep of random shapes, input row pitches and offsets, dtype
+    pairs and algorithms (through the C ABI with explicit pitches), each
+    against the oracle: integers bit-exact, floats within the bound."""
+    import ctypes as C
+
+    rng = np.random.default_rng(20261203)
+    L = sa.lib()
+    st = torch.cuda.current_stream().cuda_stream
+    for case in range(48):
+        tin, tout = PAIRS[int(rng.integers(len(PAIRS)))]
+        alg = int(rng.integers(4))
+        h, w = int(rng.integers(1, 700)), int(rng.integers(1, 1100))
+        pad, off = int(rng.integers(0, 40)), int(rng.integers(0, 8))
+        say(f"[case] random #{case}: {tin}->{tout} alg={alg} {h}x{w} pitch {w + pad + off} offset {off}")
+        big = O.gen(KIND[tin], (h, w + pad + off), 7000 + case)
+        xd = to_dev(big, dev)
+        y = torch.empty((h, w), dtype=TD[tout], device=dev)
+        esz = big.dtype.itemsize
+        st_code = L.sat_integral(alg, 0, sa.sat_dtype(TD[tin]), sa.sat_dtype(TD[tout]),
+                                 C.c_void_p(xd.data_ptr() + off * esz), w + pad + off, C.c_void_p(y.data_ptr()), w,
+                                 w, h, None, 0, C.c_void_p(st))
+        assert st_code == 0, sa.STATUS.get(st_code, st_code)
+        torch.cuda.synchronize()
+        assert_sat_close(O, np.ascontiguousarray(big[:, off:off + w]), to_np(y), tout)
