@@ -255,3 +255,37 @@ def test_random_shapes_and_pitches(O, dev, sa):
         assert st_code == 0, sa.STATUS.get(st_code, st_code)
         torch.cuda.synchronize()
         assert_sat_close(O, np.ascontiguousarray(big[:, off:off + w]), to_np(y), tout)
+
+
+def test_random_pass_shapes(O, dev, sa):
+    """Seeded random shapes for the four pass-level drop-ins (blockScan and
+    serielScan write the transposed row scan, ScanX the row scan, ScanY the
+    column scan), integer pairs bit-exact against the oracle's passes and
+    float pairs within the bound."""
+    rng = np.random.default_rng(20261204)
+    for case in range(24):
+        tin, tout = PAIRS[int(rng.integers(len(PAIRS)))]
+        h, w = int(rng.integers(1, 600)), int(rng.integers(1, 900))
+        say(f"[case] random passes #{case}: {tin}->{tout} {h}x{w}")
+        x = O.gen(KIND[tin], (h, w), 9000 + case)
+        xd = to_dev(x, dev)
+        ntout = NP[tout]
+        rows = O.allrow(x, ntout)
+        for name, got, exp, col in (
+            ("scan_x", sa.scan_x(xd, out_dtype=TD[tout]), rows, False),
+            ("scan_y", sa.scan_y(xd, out_dtype=TD[tout]), O.colscan(x, ntout), True),
+            ("block_scan", sa.block_scan(xd, out_dtype=TD[tout]), O.transpose(rows), False),
+            ("seriel_scan", sa.seriel_scan(xd, out_dtype=TD[tout]), O.transpose(rows), False),
+        ):
+            g = to_np(got)
+            assert g.shape == exp.shape, name
+            if tout in INT_OUT:
+                assert np.array_equal(g, exp), f"{name} {h}x{w}"
+                continue
+            absx = np.abs(x.astype(np.float64))
+            ref = O.colscan(x, np.float64) if col else O.allrow(x, np.float64)
+            sc = O.colscan(absx, np.float64) if col else O.allrow(absx, np.float64)
+            if name in ("block_scan", "seriel_scan"):
+                ref, sc = O.transpose(ref), O.transpose(sc)
+            u = 2.0 ** -24 if tout == "f32" else 2.0 ** -53
+            assert np.all(np.abs(g.astype(np.float64) - ref) <= 32 * u * sc + 1e-300), f"{name} {h}x{w}"
