@@ -1,11 +1,14 @@
 """Benchmark: summed-area-table throughput on one MI355X.
 
 Default workload: one-pass SAT of an f32 8192 x 8192 image (values U[0,1)),
-inputs resident in HBM. A step is one SAT of the image. Throughput is
+inputs resident in HBM. A step is one SAT of one image. Throughput is
 algorithmic bytes (H*W*(sizeof(in) + sizeof(out))) per second; the roofline is
-MI355X's 8 TB/s HBM peak. The reference's CPU integral (the oracle's
-restatement of IntegralImageParallel / IntegralImageSerial) is timed on the
-host cores of the same machine as the CPU baseline.
+MI355X's 8 TB/s HBM peak. Consecutive steps rotate over enough distinct
+input/output pairs (at least 2 GiB in all) that no step finds its input in the
+256 MiB MALL left there by an earlier step; the back-to-back rate on one pair
+is reported beside it. The reference's CPU integral (the oracle's restatement
+of IntegralImageParallel / IntegralImageSerial) is timed on the host cores of
+the same machine as the CPU baseline.
 
 Prints ONE JSON line. ``--gpus`` above 1 is not supported (one GPU only).
 """
@@ -13,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -22,6 +26,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+ROTATE_BYTES = 2 << 30  # bytes the rotating buffer pairs span at least
+PROFILE_WORKLOADS = ("f32_8192", "u32_8192", "f64_4096", "f32_65536x256", "f32_256x65536", "u8u32_16384",
+                     "u8u32_2048", "batched", "diff", "copy")
 
 
 def log(*a):
@@ -29,16 +36,17 @@ def log(*a):
 
 
 def time_launches(fn, steps: int, warmup: int):
-    """Average ms per call over `steps` back-to-back calls on the current stream."""
+    """Average ms per call over `steps` back-to-back calls fn(k), k = 0.., on
+    the current stream (where the library launches its kernels)."""
     import torch
 
-    for _ in range(warmup):
-        fn()
+    for k in range(warmup):
+        fn(k)
     torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()
-    for _ in range(steps):
-        fn()
+    for k in range(steps):
+        fn(warmup + k)
     e.record()
     torch.cuda.synchronize()
     return s.elapsed_time(e) / steps
@@ -50,17 +58,42 @@ def time_each(fn, steps: int, before=None):
     import torch
 
     ts = []
-    for _ in range(steps):
+    for k in range(steps):
         if before is not None:
             before()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
-        fn()
+        fn(k)
         e.record()
         torch.cuda.synchronize()
         ts.append(s.elapsed_time(e))
     ts.sort()
     return ts[len(ts) // 2]
+
+
+def n_rotate(pair_bytes: int) -> int:
+    """Distinct buffer pairs a rotating timing loop uses: at least 4 and at
+    least ROTATE_BYTES in all."""
+    return max(4, math.ceil(ROTATE_BYTES / max(1, pair_bytes)))
+
+
+def rotating(x, out_dtype, nbuf: int):
+    """nbuf copies of the device tensor x and nbuf outputs of its shape."""
+    import torch
+
+    xs = [x] + [x.clone() for _ in range(nbuf - 1)]
+    ys = [torch.empty(x.shape, dtype=out_dtype, device=x.device) for _ in range(nbuf)]
+    return xs, ys
+
+
+def check_faults(sat_amd, what: str) -> None:
+    """Fails the run if a look-back of the library-managed workspace of the
+    current stream gave up (its results would be invalid)."""
+    import torch
+
+    f = sat_amd.lib().sat_workspace_fault(None, torch.cuda.current_stream().cuda_stream)
+    if f != 0:
+        raise SystemExit(f"[bench] {what}: look-back fault word raised ({f}); results invalid")
 
 
 def cpu_baseline(x, tout, budget_s: float = 12.0):
@@ -96,10 +129,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--profile", action="store_true",
-                    help="profiling run: the headline SAT --steps times, then 3 plain copies of the same "
-                         "bytes (the counter calibration), no checks and no CPU baseline")
+                    help="profiling run of one workload (--profile-workload): --steps rotating calls, then "
+                         "3 plain copies of 256 MiB (the counter calibration), no checks and no CPU baseline")
+    ap.add_argument("--profile-workload", default="f32_8192", choices=PROFILE_WORKLOADS)
     ap.add_argument("--profile-alg", type=int, default=3, choices=[0, 1, 2, 3],
-                    help="algorithm of the --profile run (0-2: the reference's two-pass compositions)")
+                    help="algorithm of a --profile run of a single-image workload (0-2: the reference's "
+                         "two-pass compositions)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the SAT path here has no multi-GPU mode "
@@ -110,66 +145,66 @@ def main():
     import torch
 
     import sat_amd
-    from oracle import oracle as O
 
     dev = torch.device("cuda:0")
+    if args.profile:
+        profile_run(sat_amd, dev, args)
+        return
+
+    from oracle import oracle as O
+
     H = W = 8192
     log(f"[bench] f32 {H}x{W} one-pass SAT: generating input")
     x_np = O.gen("f32", (H, W), 20260101)
     x = torch.from_numpy(x_np).to(dev)
-    y = torch.empty_like(x)
+    algo_bytes = H * W * (4 + 4)
+    nbuf = n_rotate(algo_bytes)
+    xs, ys = rotating(x, torch.float32, nbuf)
 
-    alg = args.profile_alg if args.profile else sat_amd.SINGLE_PASS
+    def step(k):
+        sat_amd.integral(xs[k % nbuf], alg=sat_amd.SINGLE_PASS, out=ys[k % nbuf])
 
-    def step():
-        sat_amd.integral(x, alg=alg, out=y)
-
-    step()
+    step(0)
     torch.cuda.synchronize()
-    if args.profile:
-        for _ in range(args.steps):
-            step()
-        torch.cuda.synchronize()
-        y2 = torch.empty_like(x)
-        for _ in range(3):
-            y2.copy_(x)
-        lib = sat_amd.lib()
-        for _ in range(args.steps):
-            sat_amd.check(lib.sat_stream_copy(x.data_ptr(), y2.data_ptr(), H * W * 4,
-                                              torch.cuda.current_stream().cuda_stream), "sat_stream_copy")
-        torch.cuda.synchronize()
-        log(f"[bench] profile run done: {args.steps} SAT launches, 3 torch copies and {args.steps} "
-            f"sat_stream_copy calls of {H * W * 4} bytes")
-        return
     # parity of the measured output (not timed)
     log("[bench] checking the output against the FP64 oracle")
     ref = O.sat_f64(x_np)
-    got = y.cpu().numpy().astype(np.float64)
+    got = ys[0].cpu().numpy().astype(np.float64)
     rel = float(np.max(np.abs(got - ref) / np.maximum(ref, 1e-30)))
     del got, ref
 
-    log(f"[bench] timing {args.steps} steps after {args.warmup} warm-up steps")
+    log(f"[bench] timing {args.steps} steps after {args.warmup} warm-up steps, rotating over {nbuf} "
+        f"input/output pairs ({nbuf * algo_bytes / 2**30:.1f} GiB)")
     ms = time_launches(step, args.steps, args.warmup)
-    algo_bytes = H * W * (4 + 4)
+    check_faults(sat_amd, "headline")
     gbs = algo_bytes / (ms * 1e-3) / 1e9
+    ms_b2b = time_launches(lambda k: sat_amd.integral(xs[0], out=ys[0]), args.steps, args.warmup)
+    check_faults(sat_amd, "headline back-to-back")
 
     # measured ceiling: the library's streaming copy of the same bytes (the
-    # SAT's 16-byte loads and nontemporal stores without any carries)
+    # SAT's 16-byte loads and nontemporal stores without any carries), timed
+    # the same way
     log("[bench] timing the streaming-copy ceiling on the same bytes")
-    y2 = torch.empty_like(x)
     lib = sat_amd.lib()
     cstream = torch.cuda.current_stream().cuda_stream
 
-    def copy_step():
-        sat_amd.check(lib.sat_stream_copy(x.data_ptr(), y2.data_ptr(), H * W * 4, cstream), "sat_stream_copy")
+    def copy_step(k):
+        sat_amd.check(lib.sat_stream_copy(xs[k % nbuf].data_ptr(), ys[k % nbuf].data_ptr(), H * W * 4, cstream),
+                      "sat_stream_copy")
 
     cms = time_launches(copy_step, args.steps, args.warmup)
+    cms_b2b = time_launches(lambda k: copy_step(0), args.steps, args.warmup)
     copy_gbs = algo_bytes / (cms * 1e-3) / 1e9
-    del y2
+    del xs, ys
+    torch.cuda.empty_cache()
 
-    extra = {}
+    extra = {"f32_8192_SinglePass_back_to_back": {
+        "ms": round(ms_b2b, 4), "GBs": round(algo_bytes / (ms_b2b * 1e-3) / 1e9, 1),
+        "note": "one input/output pair, back-to-back (the MALL may hold part of it)"},
+        "copy_ceiling_back_to_back": {"ms": round(cms_b2b, 5),
+                                      "GBs": round(algo_bytes / (cms_b2b * 1e-3) / 1e9, 1)}}
     if not args.no_extra:
-        extra = extra_configs(O, sat_amd, dev)
+        extra.update(extra_configs(O, sat_amd, dev))
 
     cpu = None
     if not args.no_cpu:
@@ -184,6 +219,11 @@ def main():
             dts, _ = r["serial"]
             cpu["serial_1core_GBs"] = algo_bytes / dts / 1e9
             cpu["serial_1core_median_GBs"] = algo_bytes / r["serial_median"] / 1e9
+            # the faster of the two CPU paths (the reference's OpenMP column
+            # phase walks columns with a stride of a whole row)
+            cpu["fastest_cpu_path"] = ("IntegralImageSerial, 1 core" if dts < dt else
+                                       "IntegralImageParallel, all cores")
+            cpu["fastest_cpu_GBs"] = algo_bytes / min(dt, dts) / 1e9
 
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this same workload, scaled by a plain copy of known size (the committed
@@ -206,14 +246,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (splitmix64 U[0,1) pixels)",
-        "config": {"workload": "one-pass SAT, f32 8192x8192, row-major, inputs resident in HBM",
+        "config": {"workload": "one-pass SAT, f32 8192x8192, row-major, inputs resident in HBM, "
+                               f"steps rotating over {nbuf} input/output pairs "
+                               f"({nbuf * algo_bytes / 2**30:.1f} GiB)",
                    "height": H, "width": W, "algorithm": "single-pass 2-D decoupled look-back",
                    "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
                    "f32_max_rel_err_vs_fp64": rel},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": algo_bytes,
-                     "copy_ceiling": {"kernel": "sat_stream_copy, 256 MiB in + 256 MiB out",
+                     "copy_ceiling": {"kernel": "sat_stream_copy, 256 MiB in + 256 MiB out, rotating like the SAT",
                                       "GBs": round(copy_gbs, 1), "ms": round(cms, 5),
                                       "sat_frac_of_copy": round(gbs / copy_gbs, 4)}},
         "cpu_baseline": cpu,
@@ -237,27 +279,46 @@ def make_boxes(rng, planes, nbox, H, W):
     return np.stack([y0, x0, y0 + sh - 1, x0 + sw - 1], -1).astype(np.int32)
 
 
+BATCH_SEED, BATCH_N = 20260501, 512
+
+
+def batch_shapes(seed: int = BATCH_SEED, n: int = BATCH_N):
+    """(h, w, kind) of the batched workload: H and W uniform in [64, 2048],
+    kind 'u8' (u8 -> u32) or 'f32' with probability 1/2 each."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        h, w = (int(v) for v in rng.integers(64, 2049, 2))
+        out.append((h, w, "u8" if rng.integers(0, 2) == 0 else "f32"))
+    return out
+
+
+def batch_inputs(O, dev, seed: int = BATCH_SEED, n: int = BATCH_N):
+    """Host inputs, device (input, output) pairs and algorithmic bytes of the
+    batched workload."""
+    import torch
+
+    pairs, host, algo = [], [], 0
+    for i, (h, w, kind) in enumerate(batch_shapes(seed, n)):
+        x = O.gen(kind, (h, w), 20260600 + i)
+        y = torch.empty((h, w), dtype=torch.uint32 if kind == "u8" else torch.float32, device=dev)
+        algo += h * w * (5 if kind == "u8" else 8)
+        pairs.append((torch.from_numpy(x).to(dev), y))
+        host.append(x)
+    return host, pairs, algo
+
+
 def batched_config(O, sat_amd, dev, reps=10):
     """512 images, H and W uniform in [64, 2048], 50/50 u8->u32 and f32: one
-    batched launch against one plain launch per image, same buffers."""
+    batched launch against one plain launch per image on the same buffers,
+    issued from Python and replayed as one captured graph of 512 launches."""
     import numpy as np
     import torch
 
-    rng = np.random.default_rng(20260501)
-    n = 512
-    pairs, host, algo = [], [], 0
-    for i in range(n):
-        h, w = (int(v) for v in rng.integers(64, 2049, 2))
-        if rng.integers(0, 2) == 0:
-            x = O.gen("u8", (h, w), 20260600 + i)
-            y = torch.empty((h, w), dtype=torch.uint32, device=dev)
-            algo += h * w * 5
-        else:
-            x = O.gen("f32", (h, w), 20260600 + i)
-            y = torch.empty((h, w), dtype=torch.float32, device=dev)
-            algo += h * w * 8
-        pairs.append((torch.from_numpy(x).to(dev), y))
-        host.append(x)
+    host, pairs, algo = batch_inputs(O, dev)
+    n = len(pairs)
     plan = sat_amd.BatchPlan(pairs)
     plan.run()
     torch.cuda.synchronize()
@@ -273,26 +334,49 @@ def batched_config(O, sat_amd, dev, reps=10):
             ref = O.sat_f64(x)
             worst = max(worst, float(np.max(np.abs(yd.cpu().numpy() - ref) / np.maximum(ref, 1e-30))))
 
-    def per_image():
+    def per_image(k=0):
         for xd, yd in pairs:
             sat_amd.integral(xd, out=yd)
 
-    ms_b = time_launches(plan.run, reps, 2)
+    ms_b = time_launches(lambda k: plan.run(), reps, 2)
+    plan.check_faults()
     ms_p = time_launches(per_image, max(2, reps // 2), 1)
-    res = {"images": n, "algorithmic_bytes": algo, "batched_ms": round(ms_b, 4), "per_image_launch_ms": round(ms_p, 4),
-           "speedup_vs_per_image": round(ms_p / ms_b, 2), "batched_GBs": round(algo / (ms_b * 1e-3) / 1e9, 1),
+    check_faults(sat_amd, "per-image launches")
+    # the per-image path without Python: the same 512 launches captured once
+    # and replayed as one graph (each call is an epoch bump and a SAT launch)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        per_image()  # grows this stream's managed workspace to the largest image first
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        per_image()
+    ms_g = time_launches(lambda k: g.replay(), reps, 2)
+    torch.cuda.synchronize()
+    # the graph's results are the batch's results
+    for i in range(0, n, 64):
+        yd = pairs[i][1]
+        keep = yd.clone()
+        g.replay()
+        torch.cuda.synchronize()
+        if not torch.equal(yd.view(torch.int32), keep.view(torch.int32)):
+            raise AssertionError(f"graph replay of image {i} differs from the batched result")
+    res = {"images": n, "algorithmic_bytes": algo, "batched_ms": round(ms_b, 4),
+           "per_image_launch_ms_python_loop": round(ms_p, 4),
+           "per_image_launch_ms_captured_graph": round(ms_g, 4),
+           "speedup_vs_per_image_python_loop": round(ms_p / ms_b, 2),
+           "speedup_vs_per_image_captured_graph": round(ms_g / ms_b, 2),
+           "batched_GBs": round(algo / (ms_b * 1e-3) / 1e9, 1),
            "pct_peak": round(100 * algo / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 1), "f32_max_rel_err_sample": worst,
            "batch_tile_config": int(sat_amd.lib().sat_batch_tune_get(4))}
-    log(f"[bench] batched: {ms_b:.3f} ms vs per-image {ms_p:.3f} ms ({ms_p / ms_b:.2f}x)")
-    del plan, pairs
+    log(f"[bench] batched: {ms_b:.3f} ms vs per-image {ms_p:.3f} ms from Python ({ms_p / ms_b:.2f}x), "
+        f"{ms_g:.3f} ms as one graph ({ms_g / ms_b:.2f}x)")
+    del plan, pairs, g
     torch.cuda.empty_cache()
     return res
 
 
-def diff_config(O, sat_amd, dev, reps=10):
-    """N=32, C=3, 1024x1024 f32 N(0,1); box sums over 4096 boxes per plane;
-    forward = SAT + box readout, backward = the fused reverse SAT of the box
-    corners (box_filter), and the same through SAT autograd + box_sum."""
+def diff_inputs(O, dev):
     import numpy as np
     import torch
 
@@ -305,18 +389,31 @@ def diff_config(O, sat_amd, dev, reps=10):
     x = torch.from_numpy(x_np).to(dev).view(N, Cc, H, W)
     boxes = torch.from_numpy(boxes_np).to(dev)
     g = torch.from_numpy(g_np).to(dev)
+    return x_np, boxes_np, g_np, x, boxes, g
+
+
+def diff_config(O, sat_amd, dev, reps=10):
+    """N=32, C=3, 1024x1024 f32 N(0,1); box sums over 4096 boxes per plane;
+    forward = SAT + box readout, backward = the fused reverse SAT of the box
+    corners (box_filter), and the same through SAT autograd + box_sum."""
+    import numpy as np
+    import torch
+
+    x_np, boxes_np, g_np, x, boxes, g = diff_inputs(O, dev)
+    N, Cc, H, W = x.shape
+    P, nbox = boxes_np.shape[:2]
     xg = x.clone().requires_grad_(True)
 
-    def fwd():
+    def fwd(k=0):
         with torch.no_grad():
             return sat_amd.box_filter(x, boxes)
 
-    def fwd_bwd():
+    def fwd_bwd(k=0):
         xg.grad = None
         b = sat_amd.box_filter(xg, boxes)
         b.backward(g.view(b.shape))
 
-    def fwd_bwd_sat():
+    def fwd_bwd_sat(k=0):
         xg.grad = None
         b = sat_amd.box_sum(sat_amd.sat(xg), boxes)
         b.backward(g.view(b.shape))
@@ -335,40 +432,33 @@ def diff_config(O, sat_amd, dev, reps=10):
         errs_s.append(float(np.linalg.norm(S32[p].cpu().numpy().astype(np.float64) - S64) / np.linalg.norm(S64)))
         ref = O.box_sums_f64(S64, boxes_np[p][None])[0]
         errs_b.append(float(np.linalg.norm(bb[p] - ref) / np.linalg.norm(ref)))
-        gref = np.zeros((H + 1, W + 1))
-        y0, x0, y1, x1 = boxes_np[p].T
-        gp = g_np[p].astype(np.float64)
-        np.add.at(gref, (y0, x0), gp)
-        np.add.at(gref, (y0, x1 + 1), -gp)
-        np.add.at(gref, (y1 + 1, x0), -gp)
-        np.add.at(gref, (y1 + 1, x1 + 1), gp)
-        gref = np.cumsum(np.cumsum(gref, 0), 1)[:H, :W]
+        gref = box_grad_f64(boxes_np[p], g_np[p], H, W)
         errs_g.append(float(np.linalg.norm(gx[p] - gref) / np.linalg.norm(gref)))
     ms_f = time_launches(fwd, reps, 2)
     ms_fb = time_launches(fwd_bwd, reps, 2)
     ms_fb_sat = time_launches(fwd_bwd_sat, reps, 2)
+    check_faults(sat_amd, "diff")
     # comparator: torch cumsum on the same GPU
     xc = x.clone().requires_grad_(True)
 
-    def t_fwd():
+    def t_fwd(k=0):
         with torch.no_grad():
             return torch.cumsum(torch.cumsum(x, -1), -2)
 
-    def t_fwd_bwd():
+    def t_fwd_bwd(k=0):
         xc.grad = None
         y = torch.cumsum(torch.cumsum(xc, -1), -2)
         y.backward(torch.ones_like(y))
 
     ms_tf = time_launches(t_fwd, reps, 2)
     ms_tfb = time_launches(t_fwd_bwd, max(2, reps // 2), 1)
-    algo = P * H * W * 8
+    # the forward's SAT reads f32 and writes the f64 table the box sums read
+    algo = P * H * W * (4 + 8)
     res = {"planes": P, "shape": [N, Cc, H, W], "boxes_per_plane": nbox, "fwd_ms": round(ms_f, 4),
            "fwd_bwd_ms": round(ms_fb, 4), "fwd_bwd_over_fwd": round(ms_fb / ms_f, 3),
            "fwd_bwd_via_sat_autograd_ms": round(ms_fb_sat, 4),
-           "fwd_sat_GBs": round(algo / (ms_f * 1e-3) / 1e9, 1),
+           "fwd_sat_GBs_f32_in_f64_out": round(algo / (ms_f * 1e-3) / 1e9, 1),
            "sat_normwise_rel_err_f32_vs_fp64": max(errs_s), "grad_normwise_rel_err_f32_vs_fp64": max(errs_g),
-           # a box sum is a difference of four SAT values ~30x its size: the
-           # f32 SAT's rounding, amplified by that cancellation
            "box_sum_normwise_rel_err_f32_vs_fp64": max(errs_b),
            "torch_cumsum_fwd_ms": round(ms_tf, 4), "torch_cumsum_fwd_bwd_ms": round(ms_tfb, 4)}
     log(f"[bench] diff: fwd {ms_f:.3f} ms, fwd+bwd {ms_fb:.3f} ms ({ms_fb / ms_f:.2f}x), "
@@ -378,10 +468,40 @@ def diff_config(O, sat_amd, dev, reps=10):
     return res
 
 
+def box_grad_f64(boxes, g, H, W):
+    """FP64 gradient of sum_b g_b * boxsum_b(SAT(x)) w.r.t. x for one plane:
+    the suffix sums of the boxes' corner impulses."""
+    import numpy as np
+
+    gref = np.zeros((H + 1, W + 1))
+    y0, x0, y1, x1 = boxes.T
+    gp = g.astype(np.float64)
+    np.add.at(gref, (y0, x0), gp)
+    np.add.at(gref, (y0, x1 + 1), -gp)
+    np.add.at(gref, (y1 + 1, x0), -gp)
+    np.add.at(gref, (y1 + 1, x1 + 1), gp)
+    return np.cumsum(np.cumsum(gref, 0), 1)[:H, :W]
+
+
+def f64_error(O, x_np, got):
+    """Error of an f64 SAT against the oracle's f64 IntegralImageSerial (the
+    reference's summation order): max |err| in units of u * SAT(|x|) with
+    u = 2^-53, normwise relative error, and the share of bit-identical values."""
+    import numpy as np
+
+    ref = O.serial(x_np, np.float64)
+    scale = O.serial(np.abs(x_np), np.float64)
+    err = np.abs(got - ref)
+    return {"max_err_in_u_sat_abs": float(np.max(err / (2.0 ** -53 * scale + 1e-300))),
+            "normwise_rel_err": float(np.linalg.norm(err) / np.linalg.norm(ref)),
+            "bit_identical_share": float(np.mean(got == ref))}
+
+
 def extra_configs(O, sat_amd, dev):
     """Secondary workloads: the other algorithms, dtypes and shapes. Each is the
-    average of back-to-back calls (as the headline), except the 2048^2 lines:
-    those are median-timed one call at a time behind a 512 MiB write."""
+    average of calls rotating over at least 2 GiB of input/output pairs (as the
+    headline), except the 2048^2 lines: those are median-timed one call at a
+    time behind a 512 MiB write."""
     import numpy as np
     import torch
 
@@ -390,9 +510,11 @@ def extra_configs(O, sat_amd, dev):
     flush_buf = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
 
     def run(name, x, out_dtype, alg, in_b, out_b, steps=10, flush=False):
-        y = torch.empty(x.shape, dtype=out_dtype, device=dev)
-        f = lambda: sat_amd.integral(x, out_dtype=out_dtype, alg=alg, out=y)  # noqa: E731
-        f()
+        pair = x.numel() * (in_b + out_b)
+        nbuf = 1 if flush else n_rotate(pair)
+        xs, ys = rotating(x, out_dtype, nbuf)
+        f = lambda k: sat_amd.integral(xs[k % nbuf], out_dtype=out_dtype, alg=alg, out=ys[k % nbuf])  # noqa: E731
+        f(0)
         torch.cuda.synchronize()
         if flush:
             # small images fit in the 256 MiB MALL: a 512 MiB write before every
@@ -401,9 +523,14 @@ def extra_configs(O, sat_amd, dev):
             ms = time_each(f, steps, lambda: flush_buf.fill_(1))
         else:
             ms = time_launches(f, steps, 2)
-        gbs = x.numel() * (in_b + out_b) / (ms * 1e-3) / 1e9
+        check_faults(sat_amd, name)
+        gbs = pair / (ms * 1e-3) / 1e9
         res[name] = {"ms": round(ms, 4), "GBs": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 1)}
+        if nbuf > 1:
+            res[name]["rotating_pairs"] = nbuf
         log(f"[bench] {name}: {ms:.4f} ms {gbs:.1f} GB/s")
+        y = ys[0]
+        del xs, ys
         return y
 
     xf = torch.from_numpy(O.gen("f32", (8192, 8192), 20260101)).to(dev)
@@ -413,8 +540,12 @@ def extra_configs(O, sat_amd, dev):
     xu = torch.from_numpy(O.gen("u32_20", (8192, 8192), 20260102).view(np.int32)).to(dev).view(torch.uint32)
     run("u32_8192_SinglePass", xu, torch.uint32, 3, 4, 4)
     del xu
-    xd = torch.from_numpy(O.gen("f64n", (4096, 4096), 20260103)).to(dev)
-    run("f64_4096_SinglePass", xd, torch.float64, 3, 8, 8)
+    xd_np = O.gen("f64n", (4096, 4096), 20260103)
+    xd = torch.from_numpy(xd_np).to(dev)
+    for alg in (3, 0, 1, 2):
+        y = run(f"f64_4096_{sat_amd.ALG_NAMES[alg]}", xd, torch.float64, alg, 8, 8)
+        res[f"f64_4096_{sat_amd.ALG_NAMES[alg]}"]["error_vs_oracle"] = f64_error(O, xd_np, y.cpu().numpy())
+        del y
     del xd
     for h, w in ((65536, 256), (256, 65536)):
         xt = torch.from_numpy(O.gen("f32", (h, w), 20260104)).to(dev)
@@ -426,11 +557,90 @@ def extra_configs(O, sat_amd, dev):
     x2 = torch.from_numpy(O.gen("u8", (2048, 2048), 20260106)).to(dev)
     for alg in (0, 1, 2, 3):
         run(f"u8u32_2048_{sat_amd.ALG_NAMES[alg]}", x2, torch.uint32, alg, 1, 4, flush=True)
-    del x2
+    del x2, flush_buf
     torch.cuda.empty_cache()
     res["batched"] = batched_config(O, sat_amd, dev)
     res["diff"] = diff_config(O, sat_amd, dev)
     return res
+
+
+# ---------------------------------------------------------------------------
+# Profiling runs: one workload, no checks, inputs from torch's generator so
+# that no host-side oracle is needed
+# ---------------------------------------------------------------------------
+SINGLE = {  # name: (shape, input kind, output dtype name, in bytes, out bytes)
+    "f32_8192": ((8192, 8192), "f32", "float32", 4, 4),
+    "u32_8192": ((8192, 8192), "u32", "uint32", 4, 4),
+    "f64_4096": ((4096, 4096), "f64", "float64", 8, 8),
+    "f32_65536x256": ((65536, 256), "f32", "float32", 4, 4),
+    "f32_256x65536": ((256, 65536), "f32", "float32", 4, 4),
+    "u8u32_16384": ((16384, 16384), "u8", "uint32", 1, 4),
+    "u8u32_2048": ((2048, 2048), "u8", "uint32", 1, 4),
+    "copy": ((8192, 8192), "f32", "float32", 4, 4),
+}
+
+
+def _rand(kind, shape, dev):
+    import torch
+
+    g = torch.Generator(device=dev).manual_seed(20260999)
+    if kind == "f32":
+        return torch.rand(shape, generator=g, device=dev)
+    if kind == "f64":
+        return torch.randn(shape, generator=g, device=dev, dtype=torch.float64)
+    if kind == "u8":
+        return torch.randint(0, 256, shape, generator=g, device=dev, dtype=torch.uint8)
+    if kind == "u32":
+        return torch.randint(0, 1 << 20, shape, generator=g, device=dev, dtype=torch.int32).view(torch.uint32)
+    raise ValueError(kind)
+
+
+def profile_run(sat_amd, dev, args):
+    import torch
+
+    wl = args.profile_workload
+    lib = sat_amd.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+    if wl in SINGLE:
+        shape, kind, odt, ib, ob = SINGLE[wl]
+        x = _rand(kind, shape, dev)
+        nbuf = n_rotate(x.numel() * (ib + ob))
+        xs, ys = rotating(x, getattr(torch, odt), nbuf)
+        if wl == "copy":
+            fn = lambda k: sat_amd.check(lib.sat_stream_copy(  # noqa: E731
+                xs[k % nbuf].data_ptr(), ys[k % nbuf].data_ptr(), x.numel() * ib, stream), "sat_stream_copy")
+        else:
+            fn = lambda k: sat_amd.integral(xs[k % nbuf], out_dtype=getattr(torch, odt),  # noqa: E731
+                                            alg=args.profile_alg, out=ys[k % nbuf])
+        for k in range(args.steps + 1):
+            fn(k)
+        what = f"{args.steps + 1} rotating calls over {nbuf} pairs"
+    elif wl == "batched":
+        from oracle import oracle as O
+
+        _, pairs, _ = batch_inputs(O, dev)
+        plan = sat_amd.BatchPlan(pairs)
+        for _ in range(args.steps + 1):
+            plan.run()
+        what = f"{args.steps + 1} batched launches"
+    else:  # diff
+        from oracle import oracle as O
+
+        _, _, _, x, boxes, g = diff_inputs(O, dev)
+        xg = x.clone().requires_grad_(True)
+        for _ in range(args.steps + 1):
+            xg.grad = None
+            b = sat_amd.box_filter(xg, boxes)
+            b.backward(g.view(b.shape))
+        what = f"{args.steps + 1} box_filter forward + backward"
+    torch.cuda.synchronize()
+    # counter calibration: plain copies of a known size
+    c0 = torch.empty(64 << 22, dtype=torch.float32, device=dev)
+    c1 = torch.empty_like(c0)
+    for _ in range(3):
+        c1.copy_(c0)
+    torch.cuda.synchronize()
+    log(f"[bench] profile run of {wl} done: {what}, then 3 torch copies of {c0.numel() * 4} bytes")
 
 
 if __name__ == "__main__":

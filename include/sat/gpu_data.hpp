@@ -1,6 +1,7 @@
 // gpu_data.hpp - pitched 2-D / 3-D device buffers with the interface of the
-// reference's GpuData / GpuDataRef (integralImage/cudaLib.cuh:56-190), on the
-// HIP runtime, so that code written against the reference's harness
+// reference's GpuData / GpuDataRef (integralImage/cudaLib.cuh:56-190) and its
+// grid helpers UpDivide / UpRound / Align (cudaLib.cuh:25-30), on the HIP
+// runtime, so that code written against the reference's harness
 // (DevData<T> buf(w, h); buf.CopyFromHost(p, w, w, h); kernel(buf.GetData(),
 // ..., buf.DataPitch(), ...)) keeps compiling.
 //
@@ -21,10 +22,17 @@
 #ifndef SAT_AMD_GPU_DATA_HPP
 #define SAT_AMD_GPU_DATA_HPP
 
-#include <hip/hip_runtime_api.h>
+#include <hip/hip_runtime.h>
 
 #include <cstddef>
 #include <cstring>
+
+// Grid arithmetic of the reference's harnesses (cudaLib.cuh:25-30), usable on
+// the host and in kernels: Align and UpRound round x up to a multiple of y,
+// UpDivide is the number of y-sized pieces that cover x.
+__host__ __device__ __forceinline__ int Align(int x, int y) { return (x + y - 1) / y * y; }
+__host__ __device__ __forceinline__ int UpDivide(int x, int y) { return (x + y - 1) / y; }
+__host__ __device__ __forceinline__ int UpRound(int x, int y) { return UpDivide(x, y) * y; }
 
 namespace sat {
 

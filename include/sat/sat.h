@@ -37,7 +37,8 @@ typedef enum sat_status {
   SAT_ERR_DTYPE = 3,         /* dtype pair not supported by this entry point       */
   SAT_ERR_WORKSPACE = 4,     /* caller workspace too small                         */
   SAT_ERR_HIP = 5,           /* a HIP runtime call failed (launch, alloc, copy)    */
-  SAT_ERR_UNSUPPORTED = 6    /* valid request this build does not implement        */
+  SAT_ERR_UNSUPPORTED = 6,   /* valid request this build does not implement        */
+  SAT_ERR_LOOKBACK = 7       /* a look-back gave up waiting for a carry (see below) */
 } sat_status;
 
 /* Numerically equal to the reference's DataType (integralImage.h:22-32). */
@@ -77,13 +78,35 @@ int sat_abi_version(void);
 /* Size in bytes of one element of `dtype`, 0 if unknown. */
 size_t sat_dtype_size(int dtype);
 
-/* Reports and clears the fault word of a workspace (NULL: the
- * library-managed carry workspace of the calling device and `stream`): 1 if
- * some look-back since the last query gave up waiting for a carry (its
- * results are then invalid), 0 if none did, a negative sat_status on failure.
- * Synchronises `stream`. A launch never spins forever: a look-back whose
- * carry does not arrive within about a second gives up and raises the fault. */
+/* Look-back give-ups. A launch never spins forever: a look-back whose carry
+ * does not arrive within about a second gives up and the launch finishes;
+ * the results of that launch are then undefined. A give-up is recorded in
+ * two sticky places, both kept until they are read here:
+ *   - the fault word of the launch's workspace;
+ *   - a flag per device in pinned host memory.
+ * The drop-ins never fail a later call because of an earlier give-up; callers
+ * check with one of these (the Python wrappers check the device flag at every
+ * call and raise).
+ *
+ * sat_workspace_fault: reports and clears the fault word of a workspace
+ * (NULL: the library-managed carry workspaces of the calling device and
+ * `stream`): 1 if some look-back on it gave up since the last query, 0 if
+ * none did, a negative sat_status on failure. Synchronises `stream`. */
 int sat_workspace_fault(void* workspace, void* stream);
+
+/* Reads the calling device's host-visible give-up flag without
+ * synchronising: 1 if a look-back of any launch of this library on the
+ * device that has finished gave up since the flag was last cleared, else 0;
+ * clears it when `clear` is nonzero. Negative sat_status on failure. */
+int sat_device_fault(int clear);
+
+/* Frees every library-managed workspace of the calling device after
+ * synchronising it. Managed workspaces are otherwise kept until the process
+ * ends: a buffer replaced by a larger one is retired, not freed, so that
+ * graphs captured with it and launches still in flight keep valid memory.
+ * Call this only when no captured graph that used the managed workspaces
+ * will be replayed again. */
+int sat_release_workspaces(void);
 
 /* Streaming device-to-device copy of `bytes` bytes (16-byte aligned,
  * non-overlapping buffers), stream-ordered. No counterpart in the reference:
@@ -260,7 +283,11 @@ int sat_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, const void* 
 /* Gradient of sum_b g[p*nbox+b] * boxsum_b(SAT(x_p)) with respect to x:
  * grad_x[p][y][x] = sum over boxes b of plane p containing (y, x) of g[p*nbox+b].
  * Computed as the reverse SAT of the boxes' corner impulses without
- * materialising the impulse image. grad_x is overwritten. */
+ * materialising the impulse image; the reverse SAT accumulates in f64 for
+ * both dtypes and rounds once on store. grad_x is overwritten. Corner
+ * impulses that land on one pixel are first summed in `dtype` with atomics:
+ * where three or more coincide, f32 results may differ in the last bit from
+ * run to run. */
 size_t sat_box_grad_workspace_bytes(int dtype, uint32_t width, uint32_t height, uint32_t planes, uint32_t nbox);
 int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
                  int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes, void* workspace,

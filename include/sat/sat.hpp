@@ -45,6 +45,22 @@ inline size_t integral_workspace_bytes(unsigned width, unsigned height, int alg 
   return sat_integral_workspace_bytes(alg, dtype_of<TSrc>::value, dtype_of<TDst>::value, width, height);
 }
 
+// SAT_ERR_LOOKBACK if a look-back of any finished launch of this library on
+// the current device gave up since the last check (the results of that
+// launch are undefined), else SAT_OK; clears the flags it reads. With
+// `synchronize`, waits for `stream` first (through sat_workspace_fault on its
+// managed workspaces), so that every launch enqueued on it is covered.
+inline sat_status check_faults(bool synchronize = true, void* stream = nullptr) {
+  int w = 0;
+  if (synchronize) {
+    w = sat_workspace_fault(nullptr, stream);
+    if (w < 0) return (sat_status)(-w);
+  }
+  const int f = sat_device_fault(1);
+  if (f < 0) return (sat_status)(-f);
+  return (f || w) ? SAT_ERR_LOOKBACK : SAT_OK;
+}
+
 // Suffix sums (adjoint of the SAT); T is float, double, uint32_t or int32_t.
 template <typename T, int SCAN_TYPE = SAT_SCAN_KOGGE_STONE>
 inline sat_status integral_reverse(const T* in, size_t in_pitch, T* out, size_t out_pitch, unsigned width,

@@ -31,7 +31,9 @@ ALG_NAMES = {SCANROW_BRLT: "ScanRow-BRLT", BRLT_SCANROW: "BRLT-ScanRow", SCANROW
              SINGLE_PASS: "SinglePass"}
 
 STATUS = {0: "ok", 1: "null pointer", 2: "invalid argument", 3: "unsupported dtype pair",
-          4: "workspace too small", 5: "HIP runtime error", 6: "not supported by this build"}
+          4: "workspace too small", 5: "HIP runtime error", 6: "not supported by this build",
+          7: "a look-back gave up waiting for a carry: results invalid"}
+SAT_ERR_LOOKBACK = 7
 
 # every function the C ABI exports: (restype, argtypes)
 _i, _u32, _i64, _vp, _sz = C.c_int, C.c_uint32, C.c_int64, C.c_void_p, C.c_size_t
@@ -41,6 +43,8 @@ SIGNATURES = {
     "sat_dtype_pair_supported": (_i, [_i, _i]),
     "sat_status_string": (C.c_char_p, [_i]),
     "sat_workspace_fault": (_i, [_vp, _vp]),
+    "sat_device_fault": (_i, [_i]),
+    "sat_release_workspaces": (_i, []),
     "sat_stream_copy": (_i, [_vp, _vp, _sz, _vp]),
     "sat_block_scan": (_i, [_i, _i, _i, _vp, _vp, _u32, _u32, _u32, _u32, _vp]),
     "sat_seriel_scan": (_i, [_i, _i, _vp, _vp, _u32, _u32, _u32, _u32, _vp]),

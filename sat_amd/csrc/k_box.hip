@@ -50,6 +50,13 @@ template <typename T> struct GradCfg;
 template <> struct GradCfg<float> { static constexpr int NW = 4, RPT = 8; };
 template <> struct GradCfg<double> { static constexpr int NW = 4, RPT = 4; };
 
+// Reverse-SAT tile operator of the gradient. It accumulates and carries in
+// f64 for both gradient types and rounds once on store: an f32 gradient sums
+// many corner impulses of either sign, and f32 partial sums would leave
+// errors near 1e-6 of its norm.
+template <typename T>
+using GradOp = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT, double>;
+
 // Corner impulses of the boxes binned by the tile that holds them: one
 // workgroup per plane counts the impulses per tile in LDS, scans the counts
 // into offsets and places (position in tile, value) pairs, so that a gradient
@@ -63,7 +70,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
     box_bin_kernel(const int4* __restrict__ boxes, const T* __restrict__ gout, uint32_t nbox, uint32_t W, uint32_t H,
                    uint32_t nTx, uint32_t tpp, uint32_t* __restrict__ off, uint32_t* __restrict__ pos,
                    T* __restrict__ val) {
-  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  using Op = GradOp<T>;
   constexpr int TH = Op::TH, TW = Op::TW;
   __shared__ uint32_t cnt[BIN_MAX_TILES + 1];
   __shared__ uint32_t part[BIN_THREADS];
@@ -140,7 +147,7 @@ __global__ void __launch_bounds__(64 * GradCfg<T>::NW)
     box_grad_kernel(const int4* __restrict__ boxes, const T* __restrict__ gout, uint32_t nbox, ImgArgs g,
                     uint32_t tpp, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr,
                     const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos, const T* __restrict__ val) {
-  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  using Op = GradOp<T>;
   constexpr int TH = Op::TH, TW = Op::TW, NT = Op::NT, RPT = GradCfg<T>::RPT;
   union Smem {
     typename Op::Smem c;
@@ -183,7 +190,7 @@ __global__ void __launch_bounds__(64 * GradCfg<T>::NW)
       }
   }
   __syncthreads();
-  T v[RPT][4];
+  typename Op::A v[RPT][4];
   const int64_t lcol = k.pcol - k.px0;
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
@@ -239,7 +246,7 @@ sat_status launch_box_scatter(int dtype, const int32_t* boxes, uint32_t nbox, co
 
 template <typename T>
 static uint64_t grad_words(uint32_t width, uint32_t height, uint32_t planes) {
-  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  using Op = GradOp<T>;
   const uint64_t nTx = (width + Op::TW - 1) / Op::TW, nTy = (height + Op::TH - 1) / Op::TH;
   return nTx * nTy * planes * (uint64_t)(Op::VWORDS + Op::HWORDS);
 }
@@ -251,7 +258,7 @@ size_t box_grad_carry_bytes(int dtype, uint32_t width, uint32_t height, uint32_t
 
 template <typename T>
 static uint64_t grad_tiles_per_plane(uint32_t width, uint32_t height) {
-  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  using Op = GradOp<T>;
   return (uint64_t)((width + Op::TW - 1) / Op::TW) * ((height + Op::TH - 1) / Op::TH);
 }
 
@@ -269,7 +276,7 @@ template <typename T>
 static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* grad_out, void* grad_x, int64_t pitch,
                               int64_t plane_stride, uint32_t width, uint32_t height, uint32_t planes, void* ws,
                               size_t ws_bytes, void* bins, uint32_t epoch, hipStream_t stream) {
-  using Op = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>::RPT>;
+  using Op = GradOp<T>;
   const uint64_t nTx = (width + Op::TW - 1) / Op::TW, nTy = (height + Op::TH - 1) / Op::TH;
   const uint64_t tpp = nTx * nTy, ntiles = tpp * planes;
   if (ntiles == 0) return SAT_OK;
@@ -288,7 +295,8 @@ static sat_status grad_launch(const int32_t* boxes, uint32_t nbox, const void* g
   g.nTy = (uint32_t)nTy;
   g.vcarry = carries;
   g.hcarry = carries + vw;
-  g.fault = &((WsHeader*)ws)->fault;
+  g.fault.ws = &((WsHeader*)ws)->fault;
+  g.fault.host = host_fault_word();
   uint32_t *off = nullptr, *pos = nullptr;
   T* val = nullptr;
   if (bins && nbox > 0) {

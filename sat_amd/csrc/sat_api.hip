@@ -3,6 +3,7 @@
 // k_*.hip; nothing here falls back to host arithmetic.
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -178,32 +179,39 @@ int device_xcd_count() {
 // carry buffer never holds anything but zeros or tagged words: the buffer
 // holding the launch header and carries is therefore kept apart from the
 // buffer holding two-pass temporaries (raw pixel sums could otherwise pass
-// for a tag). A caller-provided workspace holds both, so its header and carry
-// area are cleared by a fill kernel before every call.
+// for a tag). A caller-provided workspace holds both, so its carry area is
+// cleared by a fill kernel before every call (its header's fault word is
+// kept until sat_workspace_fault reads it).
+//
+// A managed buffer is never freed while the library runs (only by
+// sat_release_workspaces): when a call needs a larger one, the old buffer is
+// retired and a new one, at least twice as large, takes its place. Launches
+// already enqueued, launches another host thread is about to enqueue with the
+// old pointer, and graphs captured with it therefore keep valid memory.
+// Epochs are counted per buffer, so a retired buffer's tags never repeat.
 // ---------------------------------------------------------------------------
-struct ManagedWs {
+struct Buffer {
   void* ptr = nullptr;
   size_t bytes = 0;
-  uint32_t epoch = 0;  // host epoch counter of a carry buffer
+  uint32_t epoch = 0;  // host epoch counter (carry buffers)
+};
+struct ManagedWs {
+  Buffer* cur = nullptr;
+  std::vector<Buffer*> retired;
 };
 static std::mutex g_ws_mu;
 static std::map<std::tuple<int, uintptr_t, int>, ManagedWs> g_ws;
 
 enum WsKind : int { WS_CARRY = 0, WS_TMP = 1 };
 
-static sat_status managed_buf(hipStream_t st, int kind, size_t need, void** out, ManagedWs** entry = nullptr) {
+static sat_status managed_buf(hipStream_t st, int kind, size_t need, Buffer** out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return SAT_ERR_HIP;
   std::lock_guard<std::mutex> lk(g_ws_mu);
   ManagedWs& m = g_ws[std::make_tuple(dev, (uintptr_t)st, kind)];
-  if (m.bytes < need) {
-    if (m.ptr) {
-      if (hipStreamSynchronize(st) != hipSuccess) return SAT_ERR_HIP;
-      (void)hipFree(m.ptr);
-      m.ptr = nullptr;
-      m.bytes = 0;
-    }
+  if (!m.cur || m.cur->bytes < need) {
     size_t nb = std::max(need, (size_t)1 << 20);
+    if (m.cur) nb = std::max(nb, 2 * m.cur->bytes);
     nb = (nb + 0xffff) & ~(size_t)0xffff;
     void* p = nullptr;
     if (hipMalloc(&p, nb) != hipSuccess) return SAT_ERR_HIP;
@@ -211,20 +219,48 @@ static sat_status managed_buf(hipStream_t st, int kind, size_t need, void** out,
       (void)hipFree(p);
       return SAT_ERR_HIP;
     }
-    m.ptr = p;
-    m.bytes = nb;
-    m.epoch = 0;
+    Buffer* b = new Buffer;
+    b->ptr = p;
+    b->bytes = nb;
+    if (m.cur) m.retired.push_back(m.cur);
+    m.cur = b;
   }
-  *out = m.ptr;
-  if (entry) *entry = &m;
+  *out = m.cur;
   return SAT_OK;
+}
+
+// Sticky give-up flags in pinned host memory, one word per device (see
+// satdev::Fault); allocated on first use.
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_hf_mu;
+static uint32_t* g_hf_host = nullptr;  // host view
+static uint32_t* g_hf_dev[MAX_DEVICES] = {};
+
+uint32_t* host_fault_word() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_hf_mu);
+  if (!g_hf_host) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, MAX_DEVICES * sizeof(uint32_t),
+                      hipHostMallocMapped | hipHostMallocPortable | hipHostMallocCoherent) != hipSuccess)
+      return nullptr;
+    std::memset(p, 0, MAX_DEVICES * sizeof(uint32_t));
+    g_hf_host = (uint32_t*)p;
+  }
+  if (!g_hf_dev[dev]) {
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, g_hf_host, 0) != hipSuccess) return nullptr;
+    g_hf_dev[dev] = (uint32_t*)d + dev;
+  }
+  return g_hf_dev[dev];
 }
 
 // Carry area of one API call and the epochs of its launches.
 struct CarryCtx {
   void* ws = nullptr;
   size_t bytes = 0;        // usable bytes of the area (header + carries)
-  ManagedWs* managed = nullptr;
+  Buffer* managed = nullptr;
   bool device_epoch = false;  // stream is being captured: bump on the device
   uint32_t local = 0;         // cleared caller area: next epoch
 };
@@ -238,15 +274,18 @@ static bool stream_capturing(hipStream_t st) {
 static sat_status carry_open(hipStream_t st, void* user_ws, size_t user_bytes, size_t need, CarryCtx* c) {
   if (user_ws) {
     if (user_bytes < need) return SAT_ERR_WORKSPACE;
-    const sat_status s = launch_clear(user_ws, need, st);
-    if (s != SAT_OK) return s;
+    if (need > (size_t)satdev::WS_HEADER_BYTES) {
+      const sat_status s = launch_clear((char*)user_ws + satdev::WS_HEADER_BYTES, need - satdev::WS_HEADER_BYTES, st);
+      if (s != SAT_OK) return s;
+    }
     c->ws = user_ws;
     c->bytes = need;
     c->local = 1;
     return SAT_OK;
   }
-  const sat_status s = managed_buf(st, WS_CARRY, need, &c->ws, &c->managed);
+  const sat_status s = managed_buf(st, WS_CARRY, need, &c->managed);
   if (s != SAT_OK) return s;
+  c->ws = c->managed->ptr;
   c->bytes = c->managed->bytes;
   c->device_epoch = stream_capturing(st);
   return SAT_OK;
@@ -267,7 +306,7 @@ static sat_status carry_epoch(CarryCtx& c, hipStream_t st, uint32_t* ep) {
   uint32_t e = ++c.managed->epoch;
   if (e >= satdev::EPOCH_DEVICE_BASE) {
     // host range exhausted: no stale tag may survive the restart at 1
-    const sat_status s = launch_clear(c.ws, c.bytes, st);
+    const sat_status s = launch_clear((char*)c.ws + satdev::WS_HEADER_BYTES, words * 8, st);
     if (s != SAT_OK) return s;
     e = c.managed->epoch = 1;
   }
@@ -410,8 +449,10 @@ static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_
     if (ws_bytes < P.total) return SAT_ERR_WORKSPACE;
     tmp = (char*)ws + P.tmp_off;
   } else if (P.p2.family >= 0) {
-    const sat_status s = managed_buf(st, WS_TMP, P.total - P.tmp_off, &tmp);
+    Buffer* b = nullptr;
+    const sat_status s = managed_buf(st, WS_TMP, P.total - P.tmp_off, &b);
     if (s != SAT_OK) return s;
+    tmp = b->ptr;
   }
   sat_status s = carry_open(st, ws, ws ? P.carry : 0, P.carry, &c);
   if (s != SAT_OK) return s;
@@ -544,7 +585,8 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
     it.g.nTy = nTy;
     it.g.vcarry = nullptr;  // rebased on the device address in sat_batch_prepare
     it.g.hcarry = nullptr;
-    it.g.fault = nullptr;  // set on the device address in sat_batch_prepare
+    it.g.fault.ws = nullptr;  // set on the device address in sat_batch_prepare
+    it.g.fault.host = nullptr;
     it.kind = (uint32_t)pair_id(im[i].tin, im[i].tout);
     it.tile_base = base;
     base += nTx * nTy;
@@ -573,6 +615,7 @@ const char* sat_status_string(int status) {
     case SAT_ERR_WORKSPACE: return "workspace too small";
     case SAT_ERR_HIP: return "HIP runtime error";
     case SAT_ERR_UNSUPPORTED: return "not supported by this build";
+    case SAT_ERR_LOOKBACK: return "a look-back gave up waiting for a carry: results invalid";
     default: return "unknown status";
   }
 }
@@ -614,16 +657,7 @@ int sat_batch_tune_set(int out_bytes, int cfg) {
   return SAT_OK;
 }
 
-int sat_workspace_fault(void* workspace, void* stream) {
-  hipStream_t st = (hipStream_t)stream;
-  if (!workspace) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return -SAT_ERR_HIP;
-    std::lock_guard<std::mutex> lk(g_ws_mu);
-    auto f = g_ws.find(std::make_tuple(dev, (uintptr_t)st, (int)WS_CARRY));
-    if (f == g_ws.end()) return 0;
-    workspace = f->second.ptr;
-  }
+static int read_clear_fault(void* workspace, hipStream_t st) {
   uint32_t v = 0;
   uint32_t* p = &((satdev::WsHeader*)workspace)->fault;
   if (hipStreamSynchronize(st) != hipSuccess) return -SAT_ERR_HIP;
@@ -633,6 +667,64 @@ int sat_workspace_fault(void* workspace, void* stream) {
     if (hipMemcpy(p, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess) return -SAT_ERR_HIP;
   }
   return v ? 1 : 0;
+}
+
+int sat_workspace_fault(void* workspace, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (workspace) return read_clear_fault(workspace, st);
+  // the stream's managed carry buffers: the current one and any it replaced
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -SAT_ERR_HIP;
+  std::vector<void*> bufs;
+  {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto f = g_ws.find(std::make_tuple(dev, (uintptr_t)st, (int)WS_CARRY));
+    if (f == g_ws.end() || !f->second.cur) return 0;
+    bufs.push_back(f->second.cur->ptr);
+    for (Buffer* b : f->second.retired) bufs.push_back(b->ptr);
+  }
+  int any = 0;
+  for (void* b : bufs) {
+    const int r = read_clear_fault(b, st);
+    if (r < 0) return r;
+    any |= r;
+  }
+  return any;
+}
+
+int sat_device_fault(int clear) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return -SAT_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_hf_mu);
+  if (!g_hf_host) return 0;
+  volatile uint32_t* w = g_hf_host + dev;
+  const uint32_t v = *w;
+  if (v && clear) *w = 0;
+  return v ? 1 : 0;
+}
+
+int sat_release_workspaces(void) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return SAT_ERR_HIP;
+  if (hipDeviceSynchronize() != hipSuccess) return SAT_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  for (auto it = g_ws.begin(); it != g_ws.end();) {
+    if (std::get<0>(it->first) != dev) {
+      ++it;
+      continue;
+    }
+    ManagedWs& m = it->second;
+    for (Buffer* b : m.retired) {
+      (void)hipFree(b->ptr);
+      delete b;
+    }
+    if (m.cur) {
+      (void)hipFree(m.cur->ptr);
+      delete m.cur;
+    }
+    it = g_ws.erase(it);
+  }
+  return SAT_OK;
 }
 
 int sat_stream_copy(const void* src, void* dst, size_t bytes, void* stream) {
@@ -800,7 +892,8 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
   const Geometry g = batch_geometry(info.out_size, info.cfg);
   uint64_t* carries = (uint64_t*)((char*)workspace + satdev::WS_HEADER_BYTES);
   for (auto& it : items) {
-    it.g.fault = &((satdev::WsHeader*)workspace)->fault;
+    it.g.fault.ws = &((satdev::WsHeader*)workspace)->fault;
+    it.g.fault.host = host_fault_word();
     it.g.vcarry = carries + (uint64_t)it.tile_base * g.vwords;
     it.g.hcarry = carries + (uint64_t)info.total_tiles * g.vwords + (uint64_t)it.tile_base * g.hwords;
   }
@@ -888,8 +981,10 @@ int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* gra
       bins = (char*)workspace + at;
     }
   } else if (bin_bytes) {
-    const sat_status s = managed_buf(st, WS_TMP, bin_bytes, &bins);
+    Buffer* b = nullptr;
+    const sat_status s = managed_buf(st, WS_TMP, bin_bytes, &b);
     if (s != SAT_OK) return s;
+    bins = b->ptr;
   }
   CarryCtx c;
   sat_status s = carry_open(st, workspace, workspace ? need : 0, need, &c);

@@ -21,14 +21,16 @@ template <typename T> struct arith { using type = T; };
 template <> struct arith<int32_t> { using type = uint32_t; };
 template <typename T> using arith_t = typename arith<T>::type;
 
-// TSrc -> TDst -> arithmetic type: the reference converts each pixel to TDst
-// before adding it (`sum += src[i]` with a TDst accumulator, integralImage.h:79).
-template <typename TSrc, typename TDst>
-__device__ __forceinline__ arith_t<TDst> cvt_in(TSrc x) {
-  return (arith_t<TDst>)(TDst)x;
+// TSrc -> TDst -> arithmetic type A: the reference converts each pixel to
+// TDst before adding it (`sum += src[i]` with a TDst accumulator,
+// integralImage.h:79). A is arith_t<TDst> except where a kernel accumulates
+// a narrower output in a wider type (the f32 box gradient in f64).
+template <typename TSrc, typename TDst, typename A = arith_t<TDst>>
+__device__ __forceinline__ A cvt_in(TSrc x) {
+  return (A)(TDst)x;
 }
-template <typename TDst>
-__device__ __forceinline__ TDst cvt_out(arith_t<TDst> x) {
+template <typename TDst, typename A>
+__device__ __forceinline__ TDst cvt_out(A x) {
   return (TDst)x;
 }
 
@@ -200,15 +202,29 @@ template <typename A> struct Carry {
 // a second). A launch whose carries stop arriving (a corrupted workspace, a
 // workspace shared by overlapping calls) then ends instead of hanging the
 // GPU: the give-up is recorded in the workspace header's fault word, which
-// the host reads with sat_workspace_fault().
+// the host reads with sat_workspace_fault(), and in a word of pinned host
+// memory per device, which the host reads without synchronising
+// (sat_device_fault()).
 constexpr uint32_t SPIN_LIMIT = 1u << 20;
+
+// Where a give-up is recorded: the workspace's fault word and the device's
+// host-visible sticky flag (null when the library could not map one).
+struct Fault {
+  uint32_t* ws;
+  uint32_t* host;
+};
+
+__device__ __forceinline__ void raise_fault(const Fault& f) {
+  __hip_atomic_fetch_or(f.ws, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (f.host) __hip_atomic_store(f.host, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Wait until a predecessor has published. Only the first calling lane of the
 // wave polls (its own nearest word), so a waiting wave costs one memory
 // request per poll instead of a window of words per lane; false after
-// SPIN_LIMIT polls (recorded in the fault word).
+// SPIN_LIMIT polls (recorded in the fault words).
 template <typename A>
-__device__ __forceinline__ bool wait_lead(const uint64_t* base, int64_t e, uint32_t ep4, uint32_t* fault,
+__device__ __forceinline__ bool wait_lead(const uint64_t* base, int64_t e, uint32_t ep4, const Fault& fault,
                                           uint32_t& spins) {
   const int lead = __builtin_ctzll(__ballot(1));
   for (;;) {
@@ -219,7 +235,7 @@ __device__ __forceinline__ bool wait_lead(const uint64_t* base, int64_t e, uint3
     }
     if (__builtin_amdgcn_readlane(s, lead) != 0) return true;
     if (++spins >= SPIN_LIMIT) {
-      __hip_atomic_fetch_or(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x % 64 == (uint32_t)lead) raise_fault(fault);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -235,7 +251,7 @@ __device__ __forceinline__ bool wait_lead(const uint64_t* base, int64_t e, uint3
 // seen it published.
 template <typename A, int D>
 __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t step, uint32_t n, uint32_t ep4,
-                                       uint32_t* fault) {
+                                       const Fault& fault) {
   A acc = zero<A>();
   uint32_t spins = 0;
   while (n > 0) {
@@ -273,7 +289,7 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
 // chains together, so the chains share their memory round trips.
 template <typename A, int D, int K>
 __device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t (&e0)[K], int64_t step, uint32_t n0,
-                                            uint32_t ep4, uint32_t* fault, A (&acc)[K]) {
+                                            uint32_t ep4, const Fault& fault, A (&acc)[K]) {
   int64_t e[K];
   uint32_t n[K];
   bool fin[K];
@@ -360,20 +376,19 @@ template <typename T> struct VecOut {
 // Whole chunks inside the image take one vector load; a chunk that crosses
 // the right edge or lies below the image is read element by element (zeros
 // outside).
-template <typename TSrc, typename TDst>
+template <typename TSrc, typename TDst, typename A>
 __device__ __forceinline__ void load4(const TSrc* __restrict__ row, int64_t col, int64_t W, bool row_ok,
-                                      arith_t<TDst> (&x)[4]) {
-  using A = arith_t<TDst>;
+                                      A (&x)[4]) {
   if (row_ok && col + 3 < W) {
     const typename VecG<TSrc>::type q = *(const typename VecG<TSrc>::type*)(row + col);
-    x[0] = cvt_in<TSrc, TDst>(q.x);
-    x[1] = cvt_in<TSrc, TDst>(q.y);
-    x[2] = cvt_in<TSrc, TDst>(q.z);
-    x[3] = cvt_in<TSrc, TDst>(q.w);
+    x[0] = cvt_in<TSrc, TDst, A>(q.x);
+    x[1] = cvt_in<TSrc, TDst, A>(q.y);
+    x[2] = cvt_in<TSrc, TDst, A>(q.z);
+    x[3] = cvt_in<TSrc, TDst, A>(q.w);
   } else {
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      x[c] = (row_ok && col + c < W) ? cvt_in<TSrc, TDst>(row[col + c]) : zero<A>();
+      x[c] = (row_ok && col + c < W) ? cvt_in<TSrc, TDst, A>(row[col + c]) : zero<A>();
   }
 }
 
@@ -388,22 +403,21 @@ __device__ __forceinline__ T pick4(const T (&e)[4], int i) {
   return i == 0 ? e[0] : i == 1 ? e[1] : i == 2 ? e[2] : e[3];
 }
 
-template <typename TSrc, typename TDst>
+template <typename TSrc, typename TDst, typename A>
 __device__ __forceinline__ void load4_edge(const TSrc* __restrict__ row, int64_t col, int64_t W, bool row_ok,
-                                           arith_t<TDst> (&x)[4]) {
-  using A = arith_t<TDst>;
+                                           A (&x)[4]) {
   const int64_t base = col + 3 < W ? col : W - 4;
   const typename VecG<TSrc>::type q = *(const typename VecG<TSrc>::type*)(row + base);
   const TSrc e[4] = {q.x, q.y, q.z, q.w};
   const int s = (int)(col - base);
 #pragma unroll
   for (int c = 0; c < 4; ++c)
-    x[c] = (row_ok && col + c < W) ? cvt_in<TSrc, TDst>(pick4(e, (c + s) & 3)) : zero<A>();
+    x[c] = (row_ok && col + c < W) ? cvt_in<TSrc, TDst, A>(pick4(e, (c + s) & 3)) : zero<A>();
 }
 
-template <typename TDst>
+template <typename TDst, typename A>
 __device__ __forceinline__ void store4(TDst* __restrict__ row, int64_t col, int64_t W, bool row_ok,
-                                       const arith_t<TDst> (&y)[4]) {
+                                       const A (&y)[4]) {
   if (!row_ok) return;
   if (col + 3 < W) {
     typename VecG<TDst>::type q;
@@ -458,7 +472,7 @@ struct ImgArgs {
   uint32_t nTx, nTy;  // tiles per tile-row / per tile-column
   uint64_t* vcarry;   // vertical carry words of this image's tile 0
   uint64_t* hcarry;   // horizontal carry words of this image's tile 0
-  uint32_t* fault;    // workspace fault word (look-back give-up)
+  Fault fault;        // where a look-back give-up is recorded
 };
 
 // ---------------------------------------------------------------------------

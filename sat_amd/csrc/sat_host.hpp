@@ -87,6 +87,11 @@ sat_status launch_border(size_t out_size, void* out, int64_t pitch, int64_t plan
 // Advance the header's device epoch (clearing `carry_words` on wrap), k_sat.hip.
 sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream);
 
+// Device address of the current device's sticky give-up flag in pinned host
+// memory (the host reads it without synchronising); null if it could not be
+// mapped, which only loses the host-visible copy of a give-up.
+uint32_t* host_fault_word();
+
 // Misc.
 int device_cu_count();
 size_t dtype_size(int dt);

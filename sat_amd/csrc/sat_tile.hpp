@@ -49,13 +49,15 @@ template <typename T, typename F> struct cond<false, T, F> { using type = F; };
 // MODE_ROW: row prefix sums (IncrementScan::ScanX semantics, or with TOUT the
 // transposed row scan of BlockScan::blockScan). MODE_COL: column prefix sums
 // (IncrementScan::ScanY semantics). REV mirrors both axes (suffix sums), the
-// adjoint of the forward SAT.
+// adjoint of the forward SAT. ACC is the arithmetic type (default: that of
+// TDst); a wider one accumulates and carries in it and rounds once on store.
 // ---------------------------------------------------------------------------
-template <typename TSrc, typename TDst, int MODE, bool REV, bool TOUT, int SCAN, int NW, int RPT>
+template <typename TSrc, typename TDst, int MODE, bool REV, bool TOUT, int SCAN, int NW, int RPT,
+          typename ACC = arith_t<TDst>>
 struct TileOp {
   using Src = TSrc;
   using Dst = TDst;
-  using A = arith_t<TDst>;
+  using A = ACC;
   static constexpr int TW = 256;
   static constexpr int TH = NW * RPT;
   static constexpr int NT = 64 * NW;
@@ -144,8 +146,8 @@ struct TileOp {
       for (int r = 0; r < RPT; ++r) q[r] = *(const typename VecG<TSrc>::type*)(p0 + r * step);
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
-        const A x[4] = {cvt_in<TSrc, TDst>(q[r].x), cvt_in<TSrc, TDst>(q[r].y), cvt_in<TSrc, TDst>(q[r].z),
-                        cvt_in<TSrc, TDst>(q[r].w)};
+        const A x[4] = {cvt_in<TSrc, TDst, A>(q[r].x), cvt_in<TSrc, TDst, A>(q[r].y),
+                        cvt_in<TSrc, TDst, A>(q[r].z), cvt_in<TSrc, TDst, A>(q[r].w)};
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
       }

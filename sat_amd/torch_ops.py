@@ -11,14 +11,51 @@ from typing import Sequence
 
 import torch
 
-from . import (BRLT_SCANROW, F32, F64, I8, I32, KOGGE_STONE, SCANROW_BRLT, SCANROW_COLUMN, SINGLE_PASS, U8, U32,
-               SatError, check, lib, sat_image)
+from . import (BRLT_SCANROW, F32, F64, I8, I32, KOGGE_STONE, SAT_ERR_LOOKBACK, SCANROW_BRLT, SCANROW_COLUMN,
+               SINGLE_PASS, STATUS, U8, U32, SatError, check, lib, sat_image)
 
 __all__ = [
     "sat_dtype", "default_out_dtype", "integral", "integral_bordered", "integral_reverse", "block_scan", "seriel_scan", "scan_x", "scan_y",
     "box_sums", "box_scatter", "box_grad", "BatchPlan", "sat", "box_sum", "box_filter", "SATFunction",
     "BoxSumFunction", "BoxFilterFunction", "SCANROW_BRLT", "BRLT_SCANROW", "SCANROW_COLUMN", "SINGLE_PASS",
+    "check_faults", "LookbackFault",
 ]
+
+
+class LookbackFault(SatError):
+    """A look-back of an earlier launch on this device gave up waiting for a
+    carry, so that launch's results are undefined (status SAT_ERR_LOOKBACK)."""
+
+
+def _raise_fault(where: str) -> None:
+    raise LookbackFault(f"{where}: {STATUS[SAT_ERR_LOOKBACK]} (status {SAT_ERR_LOOKBACK}); the results of the "
+                        "faulting launch and of every call that read them are undefined")
+
+
+def _guard(t: torch.Tensor) -> None:
+    """Raises if a finished launch of the library on t's device gave up in a
+    look-back since the last check: reads the library's sticky flag in pinned
+    host memory, without synchronising, and clears it."""
+    with torch.cuda.device(t.device):
+        f = lib().sat_device_fault(1)
+    if f < 0:
+        check(-f, "sat_device_fault")
+    if f:
+        _raise_fault("an earlier sat_amd call")
+
+
+def check_faults(device=None) -> None:
+    """Synchronises `device` (default: the current one) and raises
+    :class:`LookbackFault` if any look-back of this library on it gave up
+    since the last check."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    torch.cuda.synchronize(dev)
+    with torch.cuda.device(dev):
+        f = lib().sat_device_fault(1)
+    if f < 0:
+        check(-f, "sat_device_fault")
+    if f:
+        _raise_fault("check_faults")
 
 _TORCH2SAT = {torch.uint8: U8, torch.int8: I8, torch.int32: I32, torch.float32: F32, torch.float64: F64}
 if hasattr(torch, "uint32"):
@@ -48,6 +85,10 @@ def _dev_check(*ts: torch.Tensor) -> None:
     for t in ts:
         if not t.is_cuda:
             raise SatError("sat_amd operates on HIP device tensors only")
+    for t in ts[1:]:
+        if t.device != ts[0].device:
+            raise SatError("all tensors of a call must be on one device")
+    _guard(ts[0])
 
 
 def _planes(x: torch.Tensor):
@@ -75,8 +116,10 @@ def integral(x: torch.Tensor, out_dtype: torch.dtype | None = None, alg: int = S
     p, h, w = x3.shape
     if out is None:
         out = torch.empty(x.shape, dtype=out_dtype, device=x.device)
-    if out.dtype != out_dtype or out.shape != x.shape or not out.is_contiguous():
-        raise SatError("`out` must be a contiguous tensor of the input's shape and the output dtype")
+    if (not isinstance(out, torch.Tensor) or not out.is_cuda or out.device != x.device or out.dtype != out_dtype
+            or out.shape != x.shape or not out.is_contiguous()):
+        raise SatError("`out` must be a contiguous tensor of the input's shape and the output dtype, on the "
+                       "input's device")
     o3 = out.view(p, h, w)
     if x3.numel() == 0:
         return out
@@ -128,13 +171,31 @@ def _rows2d(x: torch.Tensor) -> torch.Tensor:
     return x
 
 
+def _pass_out(x: torch.Tensor, out: torch.Tensor | None, out_dtype: torch.dtype, shape: tuple) -> torch.Tensor:
+    """The output of a pass: a new tensor, or `out` once it is checked to be a
+    2-D tensor of `shape` and `out_dtype` on x's device with unit column stride
+    (the C ABI sees only a pointer and a row pitch, so a wrong `out` would be
+    written out of bounds)."""
+    if out is None:
+        return torch.empty(shape, dtype=out_dtype, device=x.device)
+    if not isinstance(out, torch.Tensor) or out.device != x.device:
+        raise SatError("`out` must be a tensor on the input's device")
+    if out.dtype != out_dtype:
+        raise SatError(f"`out` has dtype {out.dtype}, the pass writes {out_dtype}")
+    if tuple(out.shape) != tuple(shape):
+        raise SatError(f"`out` has shape {tuple(out.shape)}, the pass writes {tuple(shape)}")
+    if out.dim() != 2 or (out.numel() > 0 and (out.stride(1) != 1 or out.stride(0) < shape[1])):
+        raise SatError("`out` must be a 2-D tensor with unit column stride and rows that do not overlap")
+    return out
+
+
 def block_scan(x: torch.Tensor, out_dtype: torch.dtype | None = None, scan: int = KOGGE_STONE,
                out: torch.Tensor | None = None) -> torch.Tensor:
     """BlockScan::blockScan (ScanRow-BRLT.cu:24-25): out[x][y] = sum_{k<=x} in[y][k]."""
     x = _rows2d(x)
     h, w = x.shape
     out_dtype = out_dtype or default_out_dtype(x.dtype)
-    out = torch.empty((w, h), dtype=out_dtype, device=x.device) if out is None else out
+    out = _pass_out(x, out, out_dtype, (w, h))
     check(lib().sat_block_scan(scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), out.data_ptr(), w,
                                x.stride(0), h, out.stride(0), _stream(x)), "sat_block_scan")
     return out
@@ -145,7 +206,7 @@ def seriel_scan(x: torch.Tensor, out_dtype: torch.dtype | None = None, out: torc
     x = _rows2d(x)
     h, w = x.shape
     out_dtype = out_dtype or default_out_dtype(x.dtype)
-    out = torch.empty((w, h), dtype=out_dtype, device=x.device) if out is None else out
+    out = _pass_out(x, out, out_dtype, (w, h))
     check(lib().sat_seriel_scan(sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), out.data_ptr(), w,
                                 x.stride(0), h, out.stride(0), _stream(x)), "sat_seriel_scan")
     return out
@@ -157,7 +218,7 @@ def scan_x(x: torch.Tensor, out_dtype: torch.dtype | None = None, scan: int = KO
     x = _rows2d(x)
     h, w = x.shape
     out_dtype = out_dtype or default_out_dtype(x.dtype)
-    out = torch.empty((h, w), dtype=out_dtype, device=x.device) if out is None else out
+    out = _pass_out(x, out, out_dtype, (h, w))
     check(lib().sat_scan_rows(scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), x.stride(0),
                               out.data_ptr(), out.stride(0), w, h, _stream(x)), "sat_scan_rows")
     return out
@@ -169,7 +230,7 @@ def scan_y(x: torch.Tensor, out_dtype: torch.dtype | None = None, scan: int = KO
     x = _rows2d(x)
     h, w = x.shape
     out_dtype = out_dtype or default_out_dtype(x.dtype)
-    out = torch.empty((h, w), dtype=out_dtype, device=x.device) if out is None else out
+    out = _pass_out(x, out, out_dtype, (h, w))
     check(lib().sat_scan_cols(scan, sat_dtype(x.dtype), sat_dtype(out_dtype), x.data_ptr(), x.stride(0),
                               out.data_ptr(), out.stride(0), w, h, _stream(x)), "sat_scan_cols")
     return out
@@ -251,8 +312,20 @@ class BatchPlan:
                                       torch.cuda.current_stream(dev).cuda_stream), "sat_batch_prepare")
 
     def run(self) -> None:
+        _guard(self.workspace)
         check(lib().sat_batch_run(self.workspace.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream),
               "sat_batch_run")
+
+    def check_faults(self) -> None:
+        """Synchronises the current stream and raises :class:`LookbackFault` if
+        a look-back of a run of this plan gave up since the last check."""
+        with torch.cuda.device(self.device):
+            f = lib().sat_workspace_fault(self.workspace.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
+            lib().sat_device_fault(1)
+        if f < 0:
+            check(-f, "sat_workspace_fault")
+        if f:
+            _raise_fault("BatchPlan")
 
 
 # ---------------------------------------------------------------------------
@@ -291,14 +364,20 @@ class BoxSumFunction(torch.autograd.Function):
 
 class BoxFilterFunction(torch.autograd.Function):
     """b = box_sums(SAT(x), boxes) with a fused backward: grad_x is the reverse
-    SAT of the corner impulses, built tile by tile from the box list."""
+    SAT of the corner impulses, built tile by tile from the box list.
+
+    For f32 x the SAT the box sums read is computed in f64 (f32 in, f64 out)
+    and the sums are rounded to f32 once: a box sum is a difference of four
+    SAT values far larger than itself, so an f32 SAT would leave errors near
+    1e-6 of the sums' norm. The f64 table is internal; the gradient
+    accumulates in f64 too and is returned as f32."""
 
     @staticmethod
     def forward(ctx, x, boxes):
-        S = integral(x)
+        S = integral(x, out_dtype=torch.float64 if x.dtype == torch.float32 else None)
         ctx.save_for_backward(boxes)
         ctx.xshape = x.shape
-        return box_sums(S, boxes)
+        return box_sums(S, boxes).to(x.dtype)
 
     @staticmethod
     def backward(ctx, gb):
