@@ -353,14 +353,21 @@ def batched_config(O, sat_amd, dev, reps=10):
         per_image()
     ms_g = time_launches(lambda k: g.replay(), reps, 2)
     torch.cuda.synchronize()
-    # the graph's results are the batch's results
-    for i in range(0, n, 64):
+    # the graph's results are the batch's results: bit-identical for u8 -> u32
+    # images; f32 ones may differ in the last bits (a decoupled look-back adds
+    # a varying number of tile aggregates before it meets an inclusive prefix)
+    plan.run()
+    torch.cuda.synchronize()
+    ref = [pairs[i][1].clone() for i in range(0, n, 32)]
+    g.replay()
+    torch.cuda.synchronize()
+    for j, i in enumerate(range(0, n, 32)):
         yd = pairs[i][1]
-        keep = yd.clone()
-        g.replay()
-        torch.cuda.synchronize()
-        if not torch.equal(yd.view(torch.int32), keep.view(torch.int32)):
+        same = (torch.equal(yd.view(torch.int32), ref[j].view(torch.int32)) if yd.dtype == torch.uint32 else
+                torch.allclose(yd, ref[j], rtol=1e-5, atol=0))
+        if not same:
             raise AssertionError(f"graph replay of image {i} differs from the batched result")
+    del ref
     res = {"images": n, "algorithmic_bytes": algo, "batched_ms": round(ms_b, 4),
            "per_image_launch_ms_python_loop": round(ms_p, 4),
            "per_image_launch_ms_captured_graph": round(ms_g, 4),

@@ -173,7 +173,11 @@ def test_box_grad_caller_workspace_many_planes(O, dev, sa):
     L = sa.lib()
     nb = L.sat_box_grad_workspace_bytes(7, W, H, P, nbox)
     assert nb > L.sat_box_grad_workspace_bytes(7, W, H, P, 0)  # room for the bins
-    ws = torch.full((nb,), 0xAB, dtype=torch.uint8, device=dev)  # garbage: the library clears what it needs
+    # garbage everywhere but the 256-byte header, which a caller zero-fills
+    # once (it holds the fault word, kept across calls): the library clears
+    # the carry area it needs and overwrites the bins
+    ws = torch.full((nb,), 0xAB, dtype=torch.uint8, device=dev)
+    ws[:256].zero_()
     gx = torch.empty((P, H, W), dtype=torch.float64, device=dev)
     for rep in range(2):
         st = L.sat_box_grad(7, boxes.data_ptr(), nbox, g.data_ptr(), gx.data_ptr(), W, W * H, W, H, P,
