@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SAT_ABI_VERSION 1
+#define SAT_ABI_VERSION 2
 
 typedef enum sat_status {
   SAT_OK = 0,
@@ -234,30 +234,36 @@ int sat_batch_run(void* workspace, void* stream);
  * (BLOCK_SIZE, SMEM_COUNT, BLOCK_DIM_X/Y, COUNT_PER_THREAD, WARP_COUNT,
  * e.g. ScanRow-BRLT.cu:24, ScanRowColumn.cu:16,144), which the drop-in
  * entry points accept but do not use. Here the one-pass SAT (forward and
- * reverse) and the batch kernel pick a wave64 tile per launch:
- *   0: 4 waves, 64 x 256 tiles (32 x 256 for 8-byte outputs)
- *   1: 8 waves, 128 x 256 tiles (64 x 256)
- *   2: 8 waves, 64 x 256 tiles (32 x 256), half the registers per lane
- * from a table indexed by output width (4 or 8 bytes) and shape class
+ * reverse) and the batch kernel pick a wave64 tile per launch (tile shapes
+ * for 4-byte outputs; 8-byte outputs take half the rows):
+ *   0: 4 waves x 16 rows, 64 x 256 tiles, 4 waves per SIMD
+ *   1: 8 waves x 16 rows, 128 x 256 tiles, 4 waves per SIMD
+ *   2: 8 waves x 8 rows, 64 x 256 tiles, up to 6 waves per SIMD
+ *   3: 8 waves x 8 rows, 64 x 256 tiles, 8 waves per SIMD (64 registers)
+ *   4: 4 waves x 24 rows, 96 x 256 tiles, 3 waves per SIMD (168 registers)
+ * from a table indexed by input width (1, 4 or 8 bytes), output width (4 or
+ * 8 bytes) and shape class
  *   class_x = min(floor(log2(ceil(width / 256))), 8)
  *   class_y = min(floor(log2(ceil(height / 64))), 11)
  *   class_p = 0, 1, 2, 3 for 1, 2-3, 4-15, 16+ planes.
  * The table starts as the tuned table shipped with the build (measured by
- * sat_amd.tune on the target device and compiled in); classes it does not
- * list, and classes reset with cfg = -1, use the built-in rule (narrow images
- * of >= 4096 rows take configuration 1, the rest 0). Results of integer
- * pairs are identical in every configuration. Workspace sizes returned by the
- * *_workspace_bytes functions fit every configuration.
+ * sat_amd.tune on the target device and compiled in). A class with no entry
+ * for its input width takes the entry of its output width's same-width pair
+ * (e.g. u8 -> f64 takes f64 -> f64's), then the built-in rule (narrow images
+ * of >= 4096 rows take configuration 1, the rest 0); cfg = -1 unsets an
+ * entry. Results of integer pairs are identical in every configuration.
+ * Workspace sizes returned by the *_workspace_bytes functions fit every
+ * configuration. in_bytes must not exceed out_bytes.
  * ------------------------------------------------------------------------- */
 int sat_tile_config_count(void);
 void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class_x, int* class_y, int* class_p);
 /* configuration a class uses now (negative sat_status on bad arguments) */
-int sat_tune_get(int out_bytes, int class_x, int class_y, int class_p);
-/* cfg = -1 restores the built-in choice */
-int sat_tune_set(int out_bytes, int class_x, int class_y, int class_p, int cfg);
-/* configuration a launch of this shape would use */
-int sat_tune_pick(int out_bytes, uint32_t width, uint32_t height, uint32_t planes);
-/* batch kernel configuration (used by the next sat_batch_prepare) */
+int sat_tune_get(int in_bytes, int out_bytes, int class_x, int class_y, int class_p);
+/* cfg = -1 unsets the entry */
+int sat_tune_set(int in_bytes, int out_bytes, int class_x, int class_y, int class_p, int cfg);
+/* configuration a launch of this shape and these element widths would use */
+int sat_tune_pick(int in_bytes, int out_bytes, uint32_t width, uint32_t height, uint32_t planes);
+/* batch kernel configuration per output width (used by the next sat_batch_prepare) */
 int sat_batch_tune_get(int out_bytes);
 int sat_batch_tune_set(int out_bytes, int cfg);
 

@@ -72,9 +72,9 @@ SIGNATURES = {
                                           _sz, _vp]),
     "sat_tile_config_count": (_i, []),
     "sat_tune_class": (None, [_u32, _u32, _u32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
-    "sat_tune_get": (_i, [_i, _i, _i, _i]),
-    "sat_tune_set": (_i, [_i, _i, _i, _i, _i]),
-    "sat_tune_pick": (_i, [_i, _u32, _u32, _u32]),
+    "sat_tune_get": (_i, [_i, _i, _i, _i, _i]),
+    "sat_tune_set": (_i, [_i, _i, _i, _i, _i, _i]),
+    "sat_tune_pick": (_i, [_i, _i, _u32, _u32, _u32]),
     "sat_batch_tune_get": (_i, [_i]),
     "sat_batch_tune_set": (_i, [_i, _i]),
 }

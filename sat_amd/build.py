@@ -62,14 +62,14 @@ def write_tuned_table() -> Path:
         doc = json.loads(TUNED_JSON.read_text())
         arch = doc.get("arch")
         if arch == ARCH:
-            entries = [(int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]), int(e["class_p"]), int(e["cfg"]))
-                       for e in doc.get("entries", [])]
+            entries = [(int(e["in_bytes"]), int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]),
+                        int(e["class_p"]), int(e["cfg"])) for e in doc.get("entries", [])]
             batch = [(int(e["out_bytes"]), int(e["cfg"])) for e in doc.get("batch", [])]
     lines = [f"// Generated by sat_amd/build.py from sat_amd/{TUNED_JSON.name} (arch {arch}, build {ARCH}).",
-             "// {out_bytes, class_x, class_y, class_p, cfg}",
+             "// {in_bytes, out_bytes, class_x, class_y, class_p, cfg}",
              "static const ShippedTune k_shipped_tune[] = {"]
-    lines += [f"    {{{ob}, {cx}, {cy}, {cp}, {cfg}}}," for ob, cx, cy, cp, cfg in entries]
-    lines += ["    {0, 0, 0, 0, -1},  // end", "};", "// {out_bytes, cfg}", "static const ShippedBatch k_shipped_batch[] = {"]
+    lines += [f"    {{{ib}, {ob}, {cx}, {cy}, {cp}, {cfg}}}," for ib, ob, cx, cy, cp, cfg in entries]
+    lines += ["    {0, 0, 0, 0, 0, -1},  // end", "};", "// {out_bytes, cfg}", "static const ShippedBatch k_shipped_batch[] = {"]
     lines += [f"    {{{ob}, {cfg}}}," for ob, cfg in batch]
     lines += ["    {0, -1},  // end", "};", ""]
     text = "\n".join(lines)

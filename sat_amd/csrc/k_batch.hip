@@ -16,7 +16,8 @@ enum Kind : uint32_t {
 // SAT) for OUTB-byte outputs.
 template <int OUTB, int CFG> struct BatchCfg {
   static constexpr int NW = sathost::TileCfg<CFG>::NW;
-  static constexpr int RPT = (OUTB == 4 ? 16 : 8) / sathost::TileCfg<CFG>::RDIV;
+  static constexpr int RPT = OUTB == 4 ? sathost::TileCfg<CFG>::RPT4 : sathost::TileCfg<CFG>::RPT4 / 2;
+  static constexpr int WPE = sathost::TileCfg<CFG>::WPE;
   using Ref = typename sathost::cond_t<OUTB == 4, float, double>;
 };
 
@@ -32,7 +33,8 @@ __device__ __forceinline__ void run_kind(const ImgArgs& g, uint32_t lt, uint32_t
 // workspace's device epoch, advanced by bump_epoch_kernel before every
 // launch, so the same launch can be replayed from a captured graph.
 template <int OUTB, int CFG>
-__global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW)) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW))
+    __attribute__((amdgpu_waves_per_eu(BatchCfg<OUTB, CFG>::WPE)))
     batch_kernel(const WsHeader* hdr, const sathost::BatchItem* __restrict__ items,
                  const uint32_t* __restrict__ queue) {
   using Cfg = BatchCfg<OUTB, CFG>;
@@ -98,8 +100,11 @@ static Geometry batch_geom() {
 }
 
 Geometry batch_geometry(size_t out_size, int cfg) {
-  if (out_size == 8) return cfg == 1 ? batch_geom<8, 1>() : cfg == 2 ? batch_geom<8, 2>() : batch_geom<8, 0>();
-  return cfg == 1 ? batch_geom<4, 1>() : cfg == 2 ? batch_geom<4, 2>() : batch_geom<4, 0>();
+  if (out_size == 8)
+    return cfg == 1 ? batch_geom<8, 1>() : cfg == 2 ? batch_geom<8, 2>() : cfg == 3 ? batch_geom<8, 3>()
+         : cfg == 4 ? batch_geom<8, 4>() : batch_geom<8, 0>();
+  return cfg == 1 ? batch_geom<4, 1>() : cfg == 2 ? batch_geom<4, 2>() : cfg == 3 ? batch_geom<4, 3>()
+       : cfg == 4 ? batch_geom<4, 4>() : batch_geom<4, 0>();
 }
 
 template <int OUTB, int CFG>
@@ -115,10 +120,14 @@ sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uin
   if (out_size == 8) {
     if (cfg == 1) launch_batch_cfg<8, 1>(items, queue, blocks, ws, stream);
     else if (cfg == 2) launch_batch_cfg<8, 2>(items, queue, blocks, ws, stream);
+    else if (cfg == 3) launch_batch_cfg<8, 3>(items, queue, blocks, ws, stream);
+    else if (cfg == 4) launch_batch_cfg<8, 4>(items, queue, blocks, ws, stream);
     else launch_batch_cfg<8, 0>(items, queue, blocks, ws, stream);
   } else {
     if (cfg == 1) launch_batch_cfg<4, 1>(items, queue, blocks, ws, stream);
     else if (cfg == 2) launch_batch_cfg<4, 2>(items, queue, blocks, ws, stream);
+    else if (cfg == 3) launch_batch_cfg<4, 3>(items, queue, blocks, ws, stream);
+    else if (cfg == 4) launch_batch_cfg<4, 4>(items, queue, blocks, ws, stream);
     else launch_batch_cfg<4, 0>(items, queue, blocks, ws, stream);
   }
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;

@@ -115,6 +115,8 @@ sat_status launch_fam_sat(const Launch& L, void* ws, size_t b, hipStream_t st) {
   switch (L.cfg) {
     case 1: return launch_fam_sat_c1(L, ws, b, st);
     case 2: return launch_fam_sat_c2(L, ws, b, st);
+    case 3: return launch_fam_sat_c3(L, ws, b, st);
+    case 4: return launch_fam_sat_c4(L, ws, b, st);
     default: return launch_sat_cfg<0>(L, ws, b, st);
   }
 }
@@ -123,6 +125,8 @@ sat_status launch_fam_sat_rev(const Launch& L, void* ws, size_t b, hipStream_t s
   switch (L.cfg) {
     case 1: return launch_fam_sat_rev_c1(L, ws, b, st);
     case 2: return launch_fam_sat_rev_c2(L, ws, b, st);
+    case 3: return launch_fam_sat_rev_c3(L, ws, b, st);
+    case 4: return launch_fam_sat_rev_c4(L, ws, b, st);
     default: return launch_sat_rev_cfg<0>(L, ws, b, st);
   }
 }
@@ -135,6 +139,8 @@ Geometry geometry(int family, size_t out_size, int cfg) {
       switch (cfg) {
         case 1: return sat_geometry_cfg<1>(out_size);
         case 2: return sat_geometry_cfg<2>(out_size);
+        case 3: return sat_geometry_cfg<3>(out_size);
+        case 4: return sat_geometry_cfg<4>(out_size);
         default: return sat_geometry_cfg<0>(out_size);
       }
     case FAM_ROWS:

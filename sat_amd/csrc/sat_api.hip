@@ -61,18 +61,23 @@ static size_t carry_bytes_any_cfg(Launch L) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile-configuration table: the autotuner's choice per output width (4 or 8
-// bytes) and shape class; 0 means the built-in choice.
+// Tile-configuration table: the autotuner's choice per input width (1, 4 or
+// 8 bytes), output width (4 or 8 bytes) and shape class; 0 means unset.
 // ---------------------------------------------------------------------------
 constexpr int TUNE_CX = 9, TUNE_CY = 12, TUNE_CP = 4;
-static std::atomic<int> g_tune[2][TUNE_CX][TUNE_CY][TUNE_CP];  // cfg + 1
-static std::atomic<int> g_tune_batch[2];                        // cfg + 1
+static std::atomic<int> g_tune[3][2][TUNE_CX][TUNE_CY][TUNE_CP];  // cfg + 1
+static std::atomic<int> g_tune_batch[2];                          // cfg + 1
+
+static int in_index(int in_bytes) { return in_bytes == 1 ? 0 : in_bytes == 4 ? 1 : in_bytes == 8 ? 2 : -1; }
+static bool widths_ok(int in_bytes, int out_bytes) {
+  return in_index(in_bytes) >= 0 && (out_bytes == 4 || out_bytes == 8) && in_bytes <= out_bytes;
+}
 
 // The autotuner's shipped table (sat_amd/tuned_gfx950.json, turned into
 // tuned_table.inc by the build) is the table's content when the library is
-// loaded; sat_tune_set overrides an entry, -1 returns it to the built-in rule.
+// loaded; sat_tune_set overrides an entry, -1 returns it to unset.
 struct ShippedTune {
-  int out_bytes, cx, cy, cp, cfg;
+  int in_bytes, out_bytes, cx, cy, cp, cfg;
 };
 struct ShippedBatch {
   int out_bytes, cfg;
@@ -81,9 +86,10 @@ struct ShippedBatch {
 
 static bool load_shipped_table() {
   for (const ShippedTune& e : k_shipped_tune) {
-    if (e.cfg < 0 || e.cfg >= TILE_CFGS || (e.out_bytes != 4 && e.out_bytes != 8)) continue;
+    if (e.cfg < 0 || e.cfg >= TILE_CFGS || !widths_ok(e.in_bytes, e.out_bytes)) continue;
     if (e.cx < 0 || e.cx >= TUNE_CX || e.cy < 0 || e.cy >= TUNE_CY || e.cp < 0 || e.cp >= TUNE_CP) continue;
-    g_tune[e.out_bytes == 8 ? 1 : 0][e.cx][e.cy][e.cp].store(e.cfg + 1, std::memory_order_relaxed);
+    g_tune[in_index(e.in_bytes)][e.out_bytes == 8 ? 1 : 0][e.cx][e.cy][e.cp].store(e.cfg + 1,
+                                                                                   std::memory_order_relaxed);
   }
   for (const ShippedBatch& e : k_shipped_batch)
     if (e.cfg >= 0 && e.cfg < TILE_CFGS && (e.out_bytes == 4 || e.out_bytes == 8))
@@ -119,20 +125,23 @@ static int builtin_cfg(int cx, int cy, int cp) {
   return (cx < 3 && cy >= 6) ? 1 : 0;
 }
 
-static bool class_ok(int out_bytes, int cx, int cy, int cp) {
-  return (out_bytes == 4 || out_bytes == 8) && cx >= 0 && cx < TUNE_CX && cy >= 0 && cy < TUNE_CY && cp >= 0 &&
-         cp < TUNE_CP;
+static bool class_ok(int cx, int cy, int cp) {
+  return cx >= 0 && cx < TUNE_CX && cy >= 0 && cy < TUNE_CY && cp >= 0 && cp < TUNE_CP;
 }
 
-static int tune_lookup(int out_bytes, int cx, int cy, int cp) {
-  const int v = g_tune[out_bytes == 8 ? 1 : 0][cx][cy][cp].load(std::memory_order_relaxed);
+// An unset (input width, output width) class takes the entry of the
+// same-width pair (input width = output width), then the built-in rule.
+static int tune_lookup(int in_bytes, int out_bytes, int cx, int cy, int cp) {
+  const int o = out_bytes == 8 ? 1 : 0;
+  int v = g_tune[in_index(in_bytes)][o][cx][cy][cp].load(std::memory_order_relaxed);
+  if (v <= 0 && in_bytes != out_bytes) v = g_tune[in_index(out_bytes)][o][cx][cy][cp].load(std::memory_order_relaxed);
   return v > 0 ? v - 1 : builtin_cfg(cx, cy, cp);
 }
 
-static int tune_pick(size_t out_size, uint32_t width, uint32_t height, uint32_t planes) {
+static int tune_pick(size_t in_size, size_t out_size, uint32_t width, uint32_t height, uint32_t planes) {
   int cx, cy, cp;
   tune_class(width, height, planes, &cx, &cy, &cp);
-  return tune_lookup(out_size == 8 ? 8 : 4, cx, cy, cp);
+  return tune_lookup((int)in_size, out_size == 8 ? 8 : 4, cx, cy, cp);
 }
 
 static int batch_cfg_pick(size_t out_size) {
@@ -398,7 +407,7 @@ static sat_status plan_integral(int alg, int scan, int tin, int tout, const void
   P->tmp_plane = 0;
   switch (alg) {
     case SAT_ALG_SINGLE_PASS:
-      a.cfg = tune_pick(osz, width, height, planes);
+      a.cfg = tune_pick(dtype_size(tin), osz, width, height, planes);
       break;
     case SAT_ALG_SCANROW_BRLT:
     case SAT_ALG_BRLT_SCANROW: {
@@ -630,20 +639,22 @@ void sat_tune_class(uint32_t width, uint32_t height, uint32_t planes, int* class
   if (class_p) *class_p = cp;
 }
 
-int sat_tune_get(int out_bytes, int class_x, int class_y, int class_p) {
-  if (!class_ok(out_bytes, class_x, class_y, class_p)) return -SAT_ERR_INVALID;
-  return tune_lookup(out_bytes, class_x, class_y, class_p);
+int sat_tune_get(int in_bytes, int out_bytes, int class_x, int class_y, int class_p) {
+  if (!widths_ok(in_bytes, out_bytes) || !class_ok(class_x, class_y, class_p)) return -SAT_ERR_INVALID;
+  return tune_lookup(in_bytes, out_bytes, class_x, class_y, class_p);
 }
 
-int sat_tune_set(int out_bytes, int class_x, int class_y, int class_p, int cfg) {
-  if (!class_ok(out_bytes, class_x, class_y, class_p) || cfg < -1 || cfg >= TILE_CFGS) return SAT_ERR_INVALID;
-  g_tune[out_bytes == 8 ? 1 : 0][class_x][class_y][class_p].store(cfg + 1, std::memory_order_relaxed);
+int sat_tune_set(int in_bytes, int out_bytes, int class_x, int class_y, int class_p, int cfg) {
+  if (!widths_ok(in_bytes, out_bytes) || !class_ok(class_x, class_y, class_p) || cfg < -1 || cfg >= TILE_CFGS)
+    return SAT_ERR_INVALID;
+  g_tune[in_index(in_bytes)][out_bytes == 8 ? 1 : 0][class_x][class_y][class_p].store(cfg + 1,
+                                                                                      std::memory_order_relaxed);
   return SAT_OK;
 }
 
-int sat_tune_pick(int out_bytes, uint32_t width, uint32_t height, uint32_t planes) {
-  if (out_bytes != 4 && out_bytes != 8) return -SAT_ERR_INVALID;
-  return tune_pick((size_t)out_bytes, width, height, planes);
+int sat_tune_pick(int in_bytes, int out_bytes, uint32_t width, uint32_t height, uint32_t planes) {
+  if (!widths_ok(in_bytes, out_bytes)) return -SAT_ERR_INVALID;
+  return tune_pick((size_t)in_bytes, (size_t)out_bytes, width, height, planes);
 }
 
 int sat_batch_tune_get(int out_bytes) {
@@ -859,7 +870,7 @@ int sat_integral_reverse_planes(int scan, int dtype, const void* in, int64_t in_
   L.in_plane = in_plane_stride;
   L.out_plane = out_plane_stride;
   L.planes = planes;
-  L.cfg = tune_pick(dtype_size(dtype), width, height, planes);
+  L.cfg = tune_pick(dtype_size(dtype), dtype_size(dtype), width, height, planes);
   return run_with_ws(L, workspace, workspace_bytes, (hipStream_t)stream);
 }
 

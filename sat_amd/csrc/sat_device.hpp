@@ -286,10 +286,12 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
 
 // K independent look-back chains per lane over the same predecessors (for
 // example two columns of one tile): every round issues the windows of all K
-// chains together, so the chains share their memory round trips.
+// chains together, so the chains share their memory round trips. Chains with
+// skip[k] set read nothing and leave acc[k] = 0.
 template <typename A, int D, int K>
 __device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t (&e0)[K], int64_t step, uint32_t n0,
-                                            uint32_t ep4, const Fault& fault, A (&acc)[K]) {
+                                            uint32_t ep4, const Fault& fault, A (&acc)[K],
+                                            const bool* skip = nullptr) {
   int64_t e[K];
   uint32_t n[K];
   bool fin[K];
@@ -298,7 +300,7 @@ __device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t 
     acc[k] = zero<A>();
     e[k] = e0[k];
     n[k] = n0;
-    fin[k] = n0 == 0;
+    fin[k] = n0 == 0 || (skip && skip[k]);
   }
   uint32_t spins = 0;
   for (;;) {

@@ -30,7 +30,7 @@ struct Geometry {
 };
 // Tile configurations of the one-pass families (FAM_SAT, FAM_SAT_REV) and of
 // the batch kernel; other families have one shape. cfg in [0, TILE_CFGS).
-constexpr int TILE_CFGS = 3;
+constexpr int TILE_CFGS = 5;
 Geometry geometry(int family, size_t out_size, int cfg = 0);
 
 // One launch of a family over `planes` images of identical shape.

@@ -31,11 +31,20 @@ template <typename TDst> constexpr int kRPT_T = kRPT<TDst> / 2;
 //   2: 8 waves x kRPT/2 rows (64 x 256 / 32 x 256): half the accumulators per lane
 template <bool C, typename T, typename F> using cond_t = typename satdev::cond<C, T, F>::type;
 
+//   3: 4 waves x 32 rows at 3 waves per SIMD (128 x 256 / 64 x 256): three
+//      workgroups per CU hold 384 KB of tile data instead of 256 KB
+//   4: 4 waves x 48 rows at 2 waves per SIMD (192 x 256 / 96 x 256): the same
+//      384 KB in two workgroups of larger tiles
+// NW waves per workgroup, RPT4 rows per wave for 4-byte outputs (half for
+// 8-byte ones), WPE waves per SIMD the register budget is sized for.
 template <int CFG> struct TileCfg;
-template <> struct TileCfg<0> { static constexpr int NW = 4, RDIV = 1; };
-template <> struct TileCfg<1> { static constexpr int NW = 8, RDIV = 1; };
-template <> struct TileCfg<2> { static constexpr int NW = 8, RDIV = 2; };
-template <int CFG, typename TDst> constexpr int cfg_rpt = kRPT<TDst> / TileCfg<CFG>::RDIV;
+template <> struct TileCfg<0> { static constexpr int NW = 4, RPT4 = 16, WPE = 4; };
+template <> struct TileCfg<1> { static constexpr int NW = 8, RPT4 = 16, WPE = 4; };
+template <> struct TileCfg<2> { static constexpr int NW = 8, RPT4 = 8, WPE = 4; };
+template <> struct TileCfg<3> { static constexpr int NW = 4, RPT4 = 32, WPE = 3; };
+template <> struct TileCfg<4> { static constexpr int NW = 4, RPT4 = 48, WPE = 2; };
+template <int CFG, typename TDst>
+constexpr int cfg_rpt = sizeof(arith_t<TDst>) == 4 ? TileCfg<CFG>::RPT4 : TileCfg<CFG>::RPT4 / 2;
 
 template <class Op>
 inline Geometry geom_of() {
@@ -55,7 +64,7 @@ inline uint64_t op_carry_words(const Launch& L) {
   return ntiles * (uint64_t)(Op::VWORDS + Op::HWORDS);
 }
 
-template <class Op>
+template <class Op, int WPE = 4>
 inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStream_t stream) {
   const uint64_t nTx = (L.width + Op::TW - 1) / Op::TW, nTy = (L.height + Op::TH - 1) / Op::TH;
   const uint64_t tpp = nTx * nTy, ntiles = tpp * L.planes;
@@ -82,7 +91,7 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   satdev::TileOrder order;
   order.nx = (uint32_t)device_xcd_count();
   order.run = (order.nx > 1 && nTx % order.nx == 0) ? (uint32_t)(nTx / order.nx) : 0u;
-  hipLaunchKernelGGL(satdev::tiles_kernel<Op>, dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, g, (uint32_t)tpp,
+  hipLaunchKernelGGL((satdev::tiles_kernel<Op, WPE>), dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, g, (uint32_t)tpp,
                      L.in_plane, L.out_plane, L.epoch, (const satdev::WsHeader*)ws, order);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
@@ -134,11 +143,15 @@ inline sat_status dispatch_pairs(const Launch& L, void* ws, size_t b, hipStream_
 template <int CFG> struct FamSatCfg {
   template <typename S, typename D, int SC> struct Fwd {
     using Op = satdev::TileOp<S, D, satdev::MODE_SAT, false, false, SC, TileCfg<CFG>::NW, cfg_rpt<CFG, D>>;
-    static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
+    static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) {
+      return launch_op<Op, TileCfg<CFG>::WPE>(L, ws, b, st);
+    }
   };
   template <typename S, typename D, int SC> struct Rev {
     using Op = satdev::TileOp<S, D, satdev::MODE_SAT, true, false, SC, TileCfg<CFG>::NW, cfg_rpt<CFG, D>>;
-    static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) { return launch_op<Op>(L, ws, b, st); }
+    static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) {
+      return launch_op<Op, TileCfg<CFG>::WPE>(L, ws, b, st);
+    }
   };
 };
 
@@ -171,5 +184,9 @@ sat_status launch_fam_sat_c1(const Launch& L, void* ws, size_t b, hipStream_t st
 sat_status launch_fam_sat_rev_c1(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_c2(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_rev_c2(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_c3(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev_c3(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_c4(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev_c4(const Launch& L, void* ws, size_t b, hipStream_t st);
 
 }  // namespace sathost

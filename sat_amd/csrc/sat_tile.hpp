@@ -79,8 +79,8 @@ struct TileOp {
   static constexpr int64_t HWORDS = DO_ROW ? (int64_t)TH * Carry<A>::WORDS : 0;  // per tile
   static_assert(!(TOUT && (REV || DO_COL)), "transposed store is a row-scan pass only");
   // the horizontal look-back wave follows KH rows per lane: rows l, l+64, ...
+  // (lanes of the last chain past the tile's rows take no part)
   static constexpr int KH = (TH + 63) / 64;
-  static_assert(TH <= 64 || TH % 64 == 0, "tile rows per horizontal lane");
   static_assert(!TOUT || TH <= 64, "one lane per tile row in the transposed store");
   static_assert(!CORNER || NW >= 2, "the 2-D pass splits look-backs over waves");
   static_assert(TW % (64 * NVW) == 0, "vertical lanes");
@@ -283,12 +283,18 @@ struct TileOp {
             }
           } else {
             int64_t e[KH];
+            bool skip[KH];
 #pragma unroll
-            for (int k = 0; k < KH; ++k) e[k] = (int64_t)(t - 1) * TH + 64 * k + lane;
-            look_back_k<A, LB_WINDOW, KH>(g.hcarry, e, TH, J, ep4, g.fault, rab);
+            for (int k = 0; k < KH; ++k) {
+              skip[k] = 64 * k + lane >= TH;
+              e[k] = (int64_t)(t - 1) * TH + (skip[k] ? 0 : 64 * k + lane);
+            }
+            look_back_k<A, LB_WINDOW, KH>(g.hcarry, e, TH, J, ep4, g.fault, rab, skip);
 #pragma unroll
             for (int k = 0; k < KH; ++k)
-              Carry<A>::put(g.hcarry, (int64_t)t * TH + 64 * k + lane, rab[k] + sm.rs[64 * k + lane], ep4 | ST_INC);
+              if (!skip[k])
+                Carry<A>::put(g.hcarry, (int64_t)t * TH + 64 * k + lane, rab[k] + sm.rs[64 * k + lane],
+                              ep4 | ST_INC);
           }
         }
         if constexpr (CORNER) {
@@ -611,10 +617,11 @@ __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nTx, Tile
   return I * nTx + q * o.run + (j % o.run);
 }
 
-// Four waves per SIMD (at most 128 registers per lane): the tile kernels are
-// latency-bound on their look-backs and need the resident tiles.
-template <class Op>
-__global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(4)))
+// WPE waves per SIMD (at most 512 / WPE registers per lane): the tile kernels
+// are latency-bound on their look-backs, and the tile data the register file
+// holds (workgroups per CU x tile bytes) bounds their throughput.
+template <class Op, int WPE>
+__global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
     tiles_kernel(ImgArgs g, uint32_t tpp, int64_t in_plane, int64_t out_plane, uint32_t ep_arg, const WsHeader* hdr,
                  TileOrder order) {
   __shared__ typename Op::Smem sm;

@@ -1,8 +1,8 @@
 """Autotuner for the tile configuration of the one-pass SAT and the batch kernel.
 
 The library picks one of ``sat_tile_config_count()`` wave64 tile
-configurations per launch from a table indexed by output width (4 or 8 bytes)
-and shape class (``sat_tune_class``). This module measures every
+configurations (tile shape and waves per SIMD) per launch from a table indexed
+by input width, output width and shape class (``sat_tune_class``). This module measures every
 configuration on the current device for representative shapes of each class,
 checks that the configurations agree (integer pairs bit-for-bit, floating
 point within the summation-order tolerance), and writes the fastest choice per
@@ -23,9 +23,16 @@ import json
 import sys
 from pathlib import Path
 
-CFG_NAMES = {0: "4 waves, 64x256 (32x256 for 8-byte outputs)",
-             1: "8 waves, 128x256 (64x256)",
-             2: "8 waves, 64x256 (32x256), half the registers per lane"}
+CFG_NAMES = {0: "4 waves x 16 rows, 64x256 (32x256 for 8-byte outputs), 4 waves per SIMD",
+             1: "8 waves x 16 rows, 128x256 (64x256), 4 waves per SIMD",
+             2: "8 waves x 8 rows, 64x256 (32x256), up to 6 waves per SIMD",
+             3: "8 waves x 8 rows, 64x256 (32x256), 8 waves per SIMD",
+             4: "4 waves x 24 rows, 96x256 (48x256), 3 waves per SIMD"}
+
+# (input bytes, output bytes) pairs the table is measured for, with the dtype
+# pair timed for each: u8 -> u32, f32 -> f32, f64 -> f64. Other pairs take
+# their output width's same-width entry.
+WIDTHS = ((1, 4), (4, 4), (8, 8))
 
 # (height, width, planes) shapes that stand for their class; the benchmark
 # workloads (f32 8192^2, u8->u32 16384^2, f64 4096^2, 65536x256, 256x65536,
@@ -47,13 +54,16 @@ def classes_of(L, width: int, height: int, planes: int):
 
 
 def reset_table(L) -> None:
-    """Every class back to the library's built-in choice."""
+    """Every class back to unset (the library's built-in choice)."""
     for ob in (4, 8):
         L.sat_batch_tune_set(ob, -1)
-        for cx in range(9):
-            for cy in range(12):
-                for cp in range(4):
-                    L.sat_tune_set(ob, cx, cy, cp, -1)
+        for ib in (1, 4, 8):
+            if ib > ob:
+                continue
+            for cx in range(9):
+                for cy in range(12):
+                    for cp in range(4):
+                        L.sat_tune_set(ib, ob, cx, cy, cp, -1)
 
 
 def apply_table(path) -> int:
@@ -65,8 +75,8 @@ def apply_table(path) -> int:
     doc = json.loads(Path(path).read_text())
     n = 0
     for e in doc.get("entries", []):
-        if L.sat_tune_set(int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]), int(e["class_p"]),
-                          int(e["cfg"])) == 0:
+        if L.sat_tune_set(int(e["in_bytes"]), int(e["out_bytes"]), int(e["class_x"]), int(e["class_y"]),
+                          int(e["class_p"]), int(e["cfg"])) == 0:
             n += 1
     for e in doc.get("batch", []):
         if L.sat_batch_tune_set(int(e["out_bytes"]), int(e["cfg"])) == 0:
@@ -74,18 +84,20 @@ def apply_table(path) -> int:
     return n
 
 
-def _time(fn, reps: int) -> float:
-    """Median microseconds of `reps` back-to-back groups of 5 calls."""
+def _time(fn, reps: int, nbuf: int = 1) -> float:
+    """Median microseconds of `reps` back-to-back groups of 5 calls fn(k),
+    k cycling over nbuf buffer sets."""
     import torch
 
-    fn()
+    fn(0)
     torch.cuda.synchronize()
-    ts = []
+    ts, k = [], 0
     for _ in range(reps):
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
         for _ in range(5):
-            fn()
+            k += 1
+            fn(k % nbuf)
         e.record()
         torch.cuda.synchronize()
         ts.append(s.elapsed_time(e) * 1000 / 5)
@@ -93,47 +105,56 @@ def _time(fn, reps: int) -> float:
     return ts[len(ts) // 2]
 
 
-def _tune_shape(L, sa, H, W, P, out_bytes, reps, log):
+# timed calls rotate over input/output sets spanning at least this many bytes,
+# so that no call finds its input in the 256 MiB MALL
+ROTATE_BYTES = 1 << 30
+
+
+def _tune_shape(L, sa, H, W, P, in_bytes, out_bytes, reps, log):
     import torch
 
     dev = torch.device("cuda:0")
-    g = torch.Generator(device="cpu").manual_seed(H * 131 + W * 7 + P)
+    g = torch.Generator(device="cpu").manual_seed(H * 131 + W * 7 + P + in_bytes)
     cx, cy, cp = classes_of(L, W, H, P)
-    if out_bytes == 4:
-        # timing on f32, invariance on u8 -> u32 (bit-exact in every configuration)
-        x = torch.rand((P, H, W), generator=g).to(dev)
-        xi = torch.randint(0, 256, (P, H, W), generator=g, dtype=torch.uint8).to(dev)
-        out_dt, int_dt = torch.float32, torch.uint32
+    shape = (P, H, W)
+    if (in_bytes, out_bytes) == (1, 4):
+        # u8 -> u32: timed and compared bit for bit
+        x = torch.randint(0, 256, shape, generator=g, dtype=torch.uint8).to(dev)
+        out_dt, exact = torch.uint32, True
+    elif out_bytes == 4:
+        x = torch.rand(shape, generator=g).to(dev)
+        out_dt, exact = torch.float32, False
     else:
-        x = torch.randn((P, H, W), generator=g, dtype=torch.float64).to(dev)
-        xi = torch.randint(-(1 << 20), 1 << 20, (P, H, W), generator=g, dtype=torch.int32).to(dev)
-        out_dt, int_dt = torch.float64, torch.float64  # i32 -> f64 is exact at these magnitudes
-    y = torch.empty(x.shape, dtype=out_dt, device=dev)
-    yi = torch.empty(x.shape, dtype=int_dt, device=dev)
-    times, ref, ref_f = [], None, None
+        # integer values: every partial sum is exact in f64
+        x = torch.randint(-(1 << 20), 1 << 20, shape, generator=g, dtype=torch.int32).to(dev).double()
+        out_dt, exact = torch.float64, True
+    nbuf = max(1, min(16, -(-ROTATE_BYTES // (x.numel() * (in_bytes + out_bytes)))))
+    xs = [x] + [x.clone() for _ in range(nbuf - 1)]
+    ys = [torch.empty(shape, dtype=out_dt, device=dev) for _ in range(nbuf)]
+    times, ref = [], None
     for cfg in range(L.sat_tile_config_count()):
-        sa.check(L.sat_tune_set(out_bytes, cx, cy, cp, cfg), "sat_tune_set")
-        assert L.sat_tune_pick(out_bytes, W, H, P) == cfg
-        sa.integral(xi, out_dtype=int_dt, out=yi)
-        sa.integral(x, out=y)
+        sa.check(L.sat_tune_set(in_bytes, out_bytes, cx, cy, cp, cfg), "sat_tune_set")
+        assert L.sat_tune_pick(in_bytes, out_bytes, W, H, P) == cfg
+        sa.integral(x, out_dtype=out_dt, out=ys[0])
         torch.cuda.synchronize()
-        got = yi.view(torch.int32) if int_dt == torch.uint32 else yi
+        got = ys[0].view(torch.int32) if out_dt == torch.uint32 else ys[0]
         if ref is None:
-            ref, ref_f = got.clone(), y.clone()
-        else:
+            ref = got.clone()
+        elif exact:
             if not torch.equal(got, ref):
-                raise AssertionError(f"tile configuration {cfg} changes integer results at {P}x{H}x{W}")
+                raise AssertionError(f"tile configuration {cfg} changes exact results at {P}x{H}x{W}")
+        else:
             scale = torch.cumsum(torch.cumsum(x.abs().double(), -1), -2)
-            u = 2.0 ** -24 if out_bytes == 4 else 2.0 ** -53
-            err = float(((y.double() - ref_f.double()).abs() / scale.clamp_min(1e-300)).max())
-            if err > 64 * u:
+            err = float(((got.double() - ref.double()).abs() / scale.clamp_min(1e-300)).max())
+            if err > 64 * 2.0 ** -24:
                 raise AssertionError(f"tile configuration {cfg} differs by {err:.2e} (relative) at {P}x{H}x{W}")
-        times.append(_time(lambda: sa.integral(x, out=y), reps))
+        times.append(_time(lambda k: sa.integral(xs[k], out_dtype=out_dt, out=ys[k]), reps, nbuf))
+    sa.check_faults()
     best = min(range(len(times)), key=lambda c: times[c])
-    log(f"[tune] {P}x{H}x{W} out {out_bytes}B class ({cx},{cy},{cp}): "
+    log(f"[tune] {P}x{H}x{W} {in_bytes}->{out_bytes}B class ({cx},{cy},{cp}): "
         + "  ".join(f"cfg{c} {t:8.1f} us" for c, t in enumerate(times)) + f"  -> cfg{best}")
-    return {"out_bytes": out_bytes, "class_x": cx, "class_y": cy, "class_p": cp, "cfg": best, "shape": [H, W],
-            "planes": P, "us": [round(t, 2) for t in times]}
+    return {"in_bytes": in_bytes, "out_bytes": out_bytes, "class_x": cx, "class_y": cy, "class_p": cp, "cfg": best,
+            "shape": [H, W], "planes": P, "us": [round(t, 2) for t in times]}
 
 
 def _tune_batch(L, sa, out_bytes, n, reps, log):
@@ -167,16 +188,17 @@ def _tune_batch(L, sa, out_bytes, n, reps, log):
             for a, b in zip(outs, ints):
                 if not torch.equal(a.view(torch.int32), b.view(torch.int32)):
                     raise AssertionError(f"batch tile configuration {cfg} changes integer results")
-        times.append(_time(plan.run, reps))
+        times.append(_time(lambda k: plan.run(), reps))
     best = min(range(len(times)), key=lambda c: times[c])
     log(f"[tune] batch of {n} out {out_bytes}B: " + "  ".join(f"cfg{c} {t:8.1f} us" for c, t in enumerate(times))
         + f"  -> cfg{best}")
     return {"out_bytes": out_bytes, "cfg": best, "images": n, "us": [round(t, 2) for t in times]}
 
 
-def tune(shapes=None, out_bytes=(4, 8), batch_images=512, reps=7, log=print) -> dict:
-    """Measure every tile configuration per shape class on cuda:0 and return the
-    table (classes not covered keep the built-in choice)."""
+def tune(shapes=None, widths=WIDTHS, batch_out=(4, 8), batch_images=512, reps=7, log=print) -> dict:
+    """Measure every tile configuration per (input width, output width, shape
+    class) on cuda:0 and return the table (classes not covered keep the
+    fallbacks)."""
     import torch
 
     import sat_amd as sa
@@ -185,13 +207,13 @@ def tune(shapes=None, out_bytes=(4, 8), batch_images=512, reps=7, log=print) -> 
     shapes = FULL_SHAPES if shapes is None else shapes
     measured = {}
     try:
-        for ob in out_bytes:
+        for ib, ob in widths:
             for (H, W, P) in shapes:
-                if ob == 8 and H * W * P > (1 << 28):
-                    continue  # 8-byte outputs: keep each sweep image under 2 GiB
-                e = _tune_shape(L, sa, H, W, P, ob, reps, log)
-                measured.setdefault((ob, e["class_x"], e["class_y"], e["class_p"]), []).append(e)
-        batch = [_tune_batch(L, sa, ob, batch_images, reps, log) for ob in out_bytes] if batch_images else []
+                if H * W * P * (ib + ob) > (5 << 30):
+                    continue  # keep each sweep image pair under 5 GiB
+                e = _tune_shape(L, sa, H, W, P, ib, ob, reps, log)
+                measured.setdefault((ib, ob, e["class_x"], e["class_y"], e["class_p"]), []).append(e)
+        batch = [_tune_batch(L, sa, ob, batch_images, reps, log) for ob in batch_out] if batch_images else []
     finally:
         reset_table(L)
     torch.cuda.synchronize()
@@ -202,7 +224,8 @@ def tune(shapes=None, out_bytes=(4, 8), batch_images=512, reps=7, log=print) -> 
         ncfg = len(es[0]["us"])
         cost = [sum(e["us"][c] / min(e["us"]) for e in es) for c in range(ncfg)]
         best = min(range(ncfg), key=lambda c: cost[c])
-        entries[key] = {"out_bytes": key[0], "class_x": key[1], "class_y": key[2], "class_p": key[3], "cfg": best,
+        entries[key] = {"in_bytes": key[0], "out_bytes": key[1], "class_x": key[2], "class_y": key[3],
+                        "class_p": key[4], "cfg": best,
                         "shapes": [[e["planes"], e["shape"][0], e["shape"][1]] for e in es],
                         "us": [e["us"] for e in es]}
     return {"arch": torch.cuda.get_device_properties(0).gcnArchName.split(":")[0],
@@ -219,7 +242,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda *m: print(*m, file=sys.stderr, flush=True)  # noqa: E731
     if a.quick:
-        doc = tune(QUICK_SHAPES, out_bytes=(4,), batch_images=64, reps=a.reps, log=log)
+        doc = tune(QUICK_SHAPES, widths=((1, 4), (4, 4)), batch_out=(4,), batch_images=64, reps=a.reps, log=log)
     else:
         doc = tune(reps=a.reps, log=log)
     Path(a.out).write_text(json.dumps(doc, indent=1) + "\n")

@@ -46,7 +46,7 @@ def test_enums_match_reference(L):
     # DataType (integralImage.h:22-32), SCAN (integralImage.h:34-37)
     assert (sat_amd.U8, sat_amd.I8, sat_amd.U32, sat_amd.I32, sat_amd.F32, sat_amd.F64) == (0, 1, 4, 5, 6, 7)
     assert (sat_amd.KOGGE_STONE, sat_amd.LF) == (0, 1)
-    assert L.sat_abi_version() == 1
+    assert L.sat_abi_version() == 2
     assert [L.sat_dtype_size(d) for d in range(8)] == [1, 1, 2, 2, 4, 4, 4, 8]
     assert L.sat_status_string(4) == b"workspace too small"
 

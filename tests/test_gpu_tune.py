@@ -34,45 +34,49 @@ def sa():
     reset_table(L)
 
 
-def force(L, out_bytes, w, h, p, cfg):
+def force(L, in_bytes, out_bytes, w, h, p, cfg):
     cx, cy, cp = C.c_int(), C.c_int(), C.c_int()
     L.sat_tune_class(w, h, p, C.byref(cx), C.byref(cy), C.byref(cp))
-    assert L.sat_tune_set(out_bytes, cx.value, cy.value, cp.value, cfg) == 0
-    assert L.sat_tune_pick(out_bytes, w, h, p) == cfg
+    assert L.sat_tune_set(in_bytes, out_bytes, cx.value, cy.value, cp.value, cfg) == 0
+    assert L.sat_tune_pick(in_bytes, out_bytes, w, h, p) == cfg
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2])
+CFGS = [0, 1, 2, 3, 4]
+IN_BYTES = {"u8": 1, "i8": 1, "u32": 4, "i32": 4, "f32": 4, "f64": 8}
+
+
+@pytest.mark.parametrize("cfg", CFGS)
 @pytest.mark.parametrize("tin,tout", PAIRS)
 def test_integral_every_config(O, dev, sa, cfg, tin, tout):
     say(f"[case] cfg={cfg} {tin}->{tout}")
     L = sa.lib()
     ob = 8 if tout in ("f64",) else 4
     for k, (h, w) in enumerate(SHAPES):
-        force(L, ob, w, h, 1, cfg)
+        force(L, IN_BYTES[tin], ob, w, h, 1, cfg)
         x = O.gen(KIND[tin], (h, w), 4000 + 31 * k + cfg)
         y = sa.integral(to_dev(x, dev), out_dtype=TD[tout])
         assert_sat_close(O, x, to_np(y), tout)
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2])
+@pytest.mark.parametrize("cfg", CFGS)
 def test_planes_and_reverse_every_config(O, dev, sa, cfg):
     say(f"[case] planes + reverse cfg={cfg}")
     L = sa.lib()
     P, h, w = 5, 300, 700
     x = O.gen("u32_20", (P, h, w), 91 + cfg)
-    force(L, 4, w, h, P, cfg)
+    force(L, 4, 4, w, h, P, cfg)
     y = to_np(sa.integral(to_dev(x, dev)))
     for p in range(P):
         assert np.array_equal(y[p], O.serial(x[p], np.uint32))
     xf = O.gen("f64n", (P, h, w), 92 + cfg)
-    force(L, 8, w, h, P, cfg)
+    force(L, 8, 8, w, h, P, cfg)
     r = sa.integral_reverse(torch.from_numpy(xf).to(dev)).cpu().numpy()
     for p in range(P):
         flip = xf[p, ::-1, ::-1]
         assert_sat_close(O, np.ascontiguousarray(flip), np.ascontiguousarray(r[p, ::-1, ::-1]), "f64")
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2])
+@pytest.mark.parametrize("cfg", CFGS)
 def test_batch_every_config(O, dev, sa, cfg):
     say(f"[case] batch cfg={cfg}")
     L = sa.lib()
@@ -97,12 +101,13 @@ def test_tuner_quick(sa, tmp_path):
     say("[case] tuner loop on two small shapes")
     from sat_amd import tune
 
-    doc = tune.tune([(512, 2048, 1), (4096, 256, 1)], out_bytes=(4,), batch_images=8, reps=2,
-                    log=lambda *m: say(*m))
-    assert len(doc["entries"]) == 2 and all(0 <= e["cfg"] < 3 for e in doc["entries"])
-    assert doc["batch"][0]["cfg"] in (0, 1, 2)
+    doc = tune.tune([(512, 2048, 1), (4096, 256, 1)], widths=((1, 4), (4, 4)), batch_out=(4,), batch_images=8,
+                    reps=2, log=lambda *m: say(*m))
+    assert len(doc["entries"]) == 4 and all(0 <= e["cfg"] < 5 for e in doc["entries"])
+    assert {e["in_bytes"] for e in doc["entries"]} == {1, 4}
+    assert doc["batch"][0]["cfg"] in CFGS
     p = tmp_path / "tuned.json"
     import json
 
     p.write_text(json.dumps(doc))
-    assert tune.apply_table(p) == 3
+    assert tune.apply_table(p) == 5
