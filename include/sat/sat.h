@@ -241,6 +241,9 @@ int sat_batch_run(void* workspace, void* stream);
  *   2: 8 waves x 8 rows, 64 x 256 tiles, up to 6 waves per SIMD
  *   3: 8 waves x 8 rows, 64 x 256 tiles, 8 waves per SIMD (64 registers)
  *   4: 4 waves x 24 rows, 96 x 256 tiles, 3 waves per SIMD (168 registers)
+ *   5: 1-byte inputs into 4-byte outputs: 4 waves x 64 rows kept packed (4
+ *      input bytes per register), 256 x 256 tiles; other pairs as 1
+ * (the batch kernel takes configurations 0-4)
  * from a table indexed by input width (1, 4 or 8 bytes), output width (4 or
  * 8 bytes) and shape class
  *   class_x = min(floor(log2(ceil(width / 256))), 8)

@@ -117,6 +117,7 @@ sat_status launch_fam_sat(const Launch& L, void* ws, size_t b, hipStream_t st) {
     case 2: return launch_fam_sat_c2(L, ws, b, st);
     case 3: return launch_fam_sat_c3(L, ws, b, st);
     case 4: return launch_fam_sat_c4(L, ws, b, st);
+    case 5: return launch_fam_sat_c5(L, ws, b, st);
     default: return launch_sat_cfg<0>(L, ws, b, st);
   }
 }
@@ -127,21 +128,23 @@ sat_status launch_fam_sat_rev(const Launch& L, void* ws, size_t b, hipStream_t s
     case 2: return launch_fam_sat_rev_c2(L, ws, b, st);
     case 3: return launch_fam_sat_rev_c3(L, ws, b, st);
     case 4: return launch_fam_sat_rev_c4(L, ws, b, st);
+    case 5: return launch_fam_sat_rev_c5(L, ws, b, st);
     default: return launch_sat_rev_cfg<0>(L, ws, b, st);
   }
 }
 
-Geometry geometry(int family, size_t out_size, int cfg) {
+Geometry geometry(int family, size_t out_size, int cfg, size_t in_size) {
   const bool b8 = out_size == 8;
   switch (family) {
     case FAM_SAT:
     case FAM_SAT_REV:
       switch (cfg) {
-        case 1: return sat_geometry_cfg<1>(out_size);
-        case 2: return sat_geometry_cfg<2>(out_size);
-        case 3: return sat_geometry_cfg<3>(out_size);
-        case 4: return sat_geometry_cfg<4>(out_size);
-        default: return sat_geometry_cfg<0>(out_size);
+        case 1: return sat_geometry_cfg<1>(out_size, in_size);
+        case 2: return sat_geometry_cfg<2>(out_size, in_size);
+        case 3: return sat_geometry_cfg<3>(out_size, in_size);
+        case 4: return sat_geometry_cfg<4>(out_size, in_size);
+        case 5: return sat_geometry_cfg<5>(out_size, in_size);
+        default: return sat_geometry_cfg<0>(out_size, in_size);
       }
     case FAM_ROWS:
       return b8 ? geom_of<TileOp<double, double, MODE_ROW, false, false, 0, kNW, kRPT<double>>>()

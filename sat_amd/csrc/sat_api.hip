@@ -41,7 +41,7 @@ static uint64_t tiles_of(const Geometry& g, uint32_t width, uint32_t height, uin
 }
 
 size_t carry_bytes(const Launch& L) {
-  const Geometry g = geometry(L.family, dtype_size(L.tout), L.cfg);
+  const Geometry g = geometry(L.family, dtype_size(L.tout), L.cfg, dtype_size(L.tin));
   const uint64_t n = tiles_of(g, L.width, L.height, L.planes);
   return (size_t)(satdev::WS_HEADER_BYTES + n * (uint64_t)(g.vwords + g.hwords) * 8);
 }
@@ -92,7 +92,7 @@ static bool load_shipped_table() {
                                                                                    std::memory_order_relaxed);
   }
   for (const ShippedBatch& e : k_shipped_batch)
-    if (e.cfg >= 0 && e.cfg < TILE_CFGS && (e.out_bytes == 4 || e.out_bytes == 8))
+    if (e.cfg >= 0 && e.cfg < BATCH_CFGS && (e.out_bytes == 4 || e.out_bytes == 8))
       g_tune_batch[e.out_bytes == 8 ? 1 : 0].store(e.cfg + 1, std::memory_order_relaxed);
   return true;
 }
@@ -663,7 +663,7 @@ int sat_batch_tune_get(int out_bytes) {
 }
 
 int sat_batch_tune_set(int out_bytes, int cfg) {
-  if ((out_bytes != 4 && out_bytes != 8) || cfg < -1 || cfg >= TILE_CFGS) return SAT_ERR_INVALID;
+  if ((out_bytes != 4 && out_bytes != 8) || cfg < -1 || cfg >= BATCH_CFGS) return SAT_ERR_INVALID;
   g_tune_batch[out_bytes == 8 ? 1 : 0].store(cfg + 1, std::memory_order_relaxed);
   return SAT_OK;
 }
@@ -884,7 +884,7 @@ size_t sat_batch_workspace_bytes(const sat_image* images, int count) {
   // the largest layout over the tile configurations: valid whatever the
   // tuning table picks at sat_batch_prepare
   size_t m = 0;
-  for (int c = 0; c < TILE_CFGS; ++c) {
+  for (int c = 0; c < BATCH_CFGS; ++c) {
     BatchPlanInfo info;
     if (batch_layout(images, count, c, &info, nullptr, nullptr) != SAT_OK) return 0;
     m = std::max(m, info.total);

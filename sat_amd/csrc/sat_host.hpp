@@ -28,10 +28,13 @@ struct Geometry {
   int64_t vwords, hwords;  // carry words per tile
   int threads;
 };
-// Tile configurations of the one-pass families (FAM_SAT, FAM_SAT_REV) and of
-// the batch kernel; other families have one shape. cfg in [0, TILE_CFGS).
-constexpr int TILE_CFGS = 5;
-Geometry geometry(int family, size_t out_size, int cfg = 0);
+// Tile configurations of the one-pass families (FAM_SAT, FAM_SAT_REV), cfg in
+// [0, TILE_CFGS), and of the batch kernel, cfg in [0, BATCH_CFGS); other
+// families have one shape. The one-pass geometry of configuration 5 depends
+// on the input width.
+constexpr int TILE_CFGS = 6;
+constexpr int BATCH_CFGS = 5;
+Geometry geometry(int family, size_t out_size, int cfg = 0, size_t in_size = 4);
 
 // One launch of a family over `planes` images of identical shape.
 struct Launch {

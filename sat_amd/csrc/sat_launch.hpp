@@ -29,20 +29,26 @@ template <typename TDst> constexpr int kRPT_T = kRPT<TDst> / 2;
 //   1: 8 waves x kRPT rows  (128 x 256 / 64 x 256): half the carry words per pixel
 //      in the vertical direction, for tall images
 //   2: 8 waves x kRPT/2 rows (64 x 256 / 32 x 256): half the accumulators per lane
-template <bool C, typename T, typename F> using cond_t = typename satdev::cond<C, T, F>::type;
-
-//   3: 4 waves x 32 rows at 3 waves per SIMD (128 x 256 / 64 x 256): three
-//      workgroups per CU hold 384 KB of tile data instead of 256 KB
-//   4: 4 waves x 48 rows at 2 waves per SIMD (192 x 256 / 96 x 256): the same
-//      384 KB in two workgroups of larger tiles
+//   3: 8 waves x 8 rows at 8 waves per SIMD (64 registers): configuration 2
+//      with twice the resident waves
+//   4: 4 waves x 24 rows at 3 waves per SIMD (96 x 256 / 48 x 256): three
+//      workgroups per CU hold 288 KB of tile data instead of 256 KB
 // NW waves per workgroup, RPT4 rows per wave for 4-byte outputs (half for
 // 8-byte ones), WPE waves per SIMD the register budget is sized for.
+template <bool C, typename T, typename F> using cond_t = typename satdev::cond<C, T, F>::type;
+
 template <int CFG> struct TileCfg;
 template <> struct TileCfg<0> { static constexpr int NW = 4, RPT4 = 16, WPE = 4; };
 template <> struct TileCfg<1> { static constexpr int NW = 8, RPT4 = 16, WPE = 4; };
 template <> struct TileCfg<2> { static constexpr int NW = 8, RPT4 = 8, WPE = 4; };
-template <> struct TileCfg<3> { static constexpr int NW = 4, RPT4 = 32, WPE = 3; };
-template <> struct TileCfg<4> { static constexpr int NW = 4, RPT4 = 48, WPE = 2; };
+template <> struct TileCfg<3> { static constexpr int NW = 8, RPT4 = 8, WPE = 8; };
+template <> struct TileCfg<4> { static constexpr int NW = 4, RPT4 = 24, WPE = 3; };
+// 5: 1-byte inputs into 4-byte outputs take PackedTileOp (4 waves x 64 packed
+//    rows, 256 x 256 tiles); every other pair takes configuration 1's tile
+template <> struct TileCfg<5> { static constexpr int NW = 8, RPT4 = 16, WPE = 4; };
+constexpr int PACKED_CFG = 5;
+template <typename S, typename D> constexpr bool packable = sizeof(S) == 1 && sizeof(arith_t<D>) == 4;
+template <typename S, typename D, int SC> using PackedOp = satdev::PackedTileOp<S, D, SC, 4, 64>;
 template <int CFG, typename TDst>
 constexpr int cfg_rpt = sizeof(arith_t<TDst>) == 4 ? TileCfg<CFG>::RPT4 : TileCfg<CFG>::RPT4 / 2;
 
@@ -142,7 +148,8 @@ inline sat_status dispatch_pairs(const Launch& L, void* ws, size_t b, hipStream_
 // configuration lives in its own translation unit (k_sat*.hip).
 template <int CFG> struct FamSatCfg {
   template <typename S, typename D, int SC> struct Fwd {
-    using Op = satdev::TileOp<S, D, satdev::MODE_SAT, false, false, SC, TileCfg<CFG>::NW, cfg_rpt<CFG, D>>;
+    using Op = cond_t<CFG == PACKED_CFG && packable<S, D>, PackedOp<S, D, SC>,
+                      satdev::TileOp<S, D, satdev::MODE_SAT, false, false, SC, TileCfg<CFG>::NW, cfg_rpt<CFG, D>>>;
     static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) {
       return launch_op<Op, TileCfg<CFG>::WPE>(L, ws, b, st);
     }
@@ -175,8 +182,9 @@ inline sat_status launch_sat_rev_cfg(const Launch& L, void* ws, size_t b, hipStr
 }
 
 template <int CFG>
-inline Geometry sat_geometry_cfg(size_t out_size) {
+inline Geometry sat_geometry_cfg(size_t out_size, size_t in_size) {
   if (out_size == 8) return geom_of<typename FamSatCfg<CFG>::template Fwd<double, double, 0>::Op>();
+  if (in_size == 1) return geom_of<typename FamSatCfg<CFG>::template Fwd<uint8_t, uint32_t, 0>::Op>();
   return geom_of<typename FamSatCfg<CFG>::template Fwd<float, float, 0>::Op>();
 }
 
@@ -188,5 +196,7 @@ sat_status launch_fam_sat_c3(const Launch& L, void* ws, size_t b, hipStream_t st
 sat_status launch_fam_sat_rev_c3(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_c4(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_rev_c4(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_c5(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev_c5(const Launch& L, void* ws, size_t b, hipStream_t st);
 
 }  // namespace sathost

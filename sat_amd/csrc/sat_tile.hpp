@@ -175,82 +175,36 @@ struct TileOp {
     finish(g, t, ep4, sm, k, v);
   }
 
-  // Everything after the load: scans, look-backs, store. `v` holds the tile
-  // in logical order (mirrored when REV).
-  __device__ static __forceinline__ void finish(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm,
-                                                const Coords& k, A (&v)[RPT][4]) {
+  // The tile's column sums, per wave in sm.ct, summed over the waves: the
+  // first NVW waves publish them (AGG; INC on the first tile row) and keep
+  // their columns' sums in colsum. Starts with the barrier that makes sm.ct
+  // complete.
+  __device__ static __forceinline__ void publish_colsums(const ImgArgs& g, uint32_t t, uint32_t I, uint32_t ep4,
+                                                         Smem& sm, A (&colsum)[CPV]) {
+    const int tid = threadIdx.x, w = tid >> 6;
+    __syncthreads();
+    if (w < NVW) {
+#pragma unroll
+      for (int q = 0; q < CPV; ++q) {
+        const int j = q * 64 * NVW + tid;
+        A cs = sm.ct[0][j];
+#pragma unroll
+        for (int u = 1; u < NW; ++u) cs += sm.ct[u][j];
+        colsum[q] = cs;
+        Carry<A>::put(g.vcarry, (int64_t)t * VS + j, cs, ep4 | (I == 0 ? ST_INC : ST_AGG));
+      }
+    }
+  }
+
+  // The three look-backs (after the tile's column and row sums are
+  // published, sm.rs filled and a barrier passed): vertical on the first NVW
+  // waves into sm.cab, horizontal on wave HWAVE into sm.rc (row carries,
+  // prefixed over the tile's rows for the 2-D SAT, whose total is the corner
+  // slot's aggregate), the corner on wave CWAVE into sm.corner; each
+  // publishes its inclusive words. Ends after a barrier.
+  __device__ static __forceinline__ void lookbacks(const ImgArgs& g, uint32_t t, uint32_t I, uint32_t J, uint32_t ep4,
+                                                   Smem& sm, const A (&colsum)[CPV]) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t I = k.I, J = k.J;
-    const int64_t py0 = k.py0, px0 = k.px0, pcol = k.pcol;
-
-    // ---- per-wave column sums; the tile's column sums are published first ----
-    A colsum[CPV];
-    if constexpr (DO_COL) {
-      if constexpr (!DO_ROW) {
-        // column pass: the in-lane column prefix is the result within the wave
-#pragma unroll
-        for (int r = 1; r < RPT; ++r)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) v[r][c] += v[r - 1][c];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sm.ct[w][4 * lane + c] = v[RPT - 1][c];
-      } else {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          A sum = v[0][c];
-#pragma unroll
-          for (int r = 1; r < RPT; ++r) sum += v[r][c];
-          sm.ct[w][4 * lane + c] = sum;
-        }
-      }
-      __syncthreads();
-      if (w < NVW) {
-#pragma unroll
-        for (int q = 0; q < CPV; ++q) {
-          const int j = q * 64 * NVW + tid;
-          A cs = sm.ct[0][j];
-#pragma unroll
-          for (int u = 1; u < NW; ++u) cs += sm.ct[u][j];
-          colsum[q] = cs;
-          Carry<A>::put(g.vcarry, (int64_t)t * VS + j, cs, ep4 | (I == 0 ? ST_INC : ST_AGG));
-        }
-      }
-    }
-
-    // ---- row prefix: in-lane over 4 columns, then across the 64 lanes; the
-    //      tile's row sums are published as soon as they are known ----
-    if constexpr (DO_ROW) {
-      A myrs = zero<A>();  // lane r < RPT: sum of this wave's row r
-#pragma unroll
-      for (int r = 0; r < RPT; ++r) {
-        v[r][1] += v[r][0];
-        v[r][2] += v[r][1];
-        v[r][3] += v[r][2];
-        const A inc = wave_scan<SCAN>(v[r][3], lane);
-        const A exc = wave_shr1(inc);
-        const A rsum = readlane(inc, 63);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[r][c] += exc;
-        if (lane == r) myrs = rsum;
-      }
-      if (lane < RPT) {
-        // one store instruction per wave publishes its RPT row sums
-        const int i = w * RPT + lane;
-        Carry<A>::put(g.hcarry, (int64_t)t * TH + i, myrs, ep4 | (J == 0 ? ST_INC : ST_AGG));
-        sm.rs[i] = myrs;
-      }
-      if constexpr (DO_COL) {
-        // tile-local column prefix of the row prefixes: the wave-local SAT
-#pragma unroll
-        for (int r = 1; r < RPT; ++r)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) v[r][c] += v[r - 1][c];
-      }
-      __syncthreads();
-    }
-
-    // ---- look-backs: vertical on the first NVW waves, horizontal and corner
-    //      on the last wave; they run concurrently ----
     if constexpr (DO_COL) {
       if (w < NVW) {
         A acc[CPV];
@@ -337,8 +291,15 @@ struct TileOp {
         Carry<A>::put(g.vcarry, (int64_t)t * VS + TW, sm.corner + sm.ksum, ep4 | ST_INC);
     }
 
-    // ---- per-column and per-row additions ----
-    A colAdd[4] = {zero<A>(), zero<A>(), zero<A>(), zero<A>()};
+  }
+
+  // colAdd[c] for this lane's column 4*lane + c: the column sums above the
+  // tile (sm.cab) and above this wave within the tile, prefixed over the
+  // columns for the 2-D SAT.
+  __device__ static __forceinline__ void col_add(const Smem& sm, A (&colAdd)[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) colAdd[c] = zero<A>();
     if constexpr (DO_COL) {
       A U[4];
 #pragma unroll
@@ -361,6 +322,76 @@ struct TileOp {
         for (int c = 0; c < 4; ++c) colAdd[c] = U[c];
       }
     }
+  }
+
+  // Everything after the load: scans, look-backs, store. `v` holds the tile
+  // in logical order (mirrored when REV).
+  __device__ static __forceinline__ void finish(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm,
+                                                const Coords& k, A (&v)[RPT][4]) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t I = k.I, J = k.J;
+    const int64_t py0 = k.py0, px0 = k.px0, pcol = k.pcol;
+
+    // ---- per-wave column sums; the tile's column sums are published first ----
+    A colsum[CPV];
+    if constexpr (DO_COL) {
+      if constexpr (!DO_ROW) {
+        // column pass: the in-lane column prefix is the result within the wave
+#pragma unroll
+        for (int r = 1; r < RPT; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[r][c] += v[r - 1][c];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sm.ct[w][4 * lane + c] = v[RPT - 1][c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          A sum = v[0][c];
+#pragma unroll
+          for (int r = 1; r < RPT; ++r) sum += v[r][c];
+          sm.ct[w][4 * lane + c] = sum;
+        }
+      }
+      publish_colsums(g, t, I, ep4, sm, colsum);
+    }
+
+    // ---- row prefix: in-lane over 4 columns, then across the 64 lanes; the
+    //      tile's row sums are published as soon as they are known ----
+    if constexpr (DO_ROW) {
+      A myrs = zero<A>();  // lane r < RPT: sum of this wave's row r
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        v[r][1] += v[r][0];
+        v[r][2] += v[r][1];
+        v[r][3] += v[r][2];
+        const A inc = wave_scan<SCAN>(v[r][3], lane);
+        const A exc = wave_shr1(inc);
+        const A rsum = readlane(inc, 63);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[r][c] += exc;
+        if (lane == r) myrs = rsum;
+      }
+      if (lane < RPT) {
+        // one store instruction per wave publishes its RPT row sums
+        const int i = w * RPT + lane;
+        Carry<A>::put(g.hcarry, (int64_t)t * TH + i, myrs, ep4 | (J == 0 ? ST_INC : ST_AGG));
+        sm.rs[i] = myrs;
+      }
+      if constexpr (DO_COL) {
+        // tile-local column prefix of the row prefixes: the wave-local SAT
+#pragma unroll
+        for (int r = 1; r < RPT; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[r][c] += v[r - 1][c];
+      }
+      __syncthreads();
+    }
+
+    lookbacks(g, t, I, J, ep4, sm, colsum);
+
+    // ---- per-column and per-row additions ----
+    A colAdd[4];
+    col_add(sm, colAdd);
     A rp[RPT];
     if constexpr (DO_ROW) {
       A corner = zero<A>();
@@ -438,6 +469,166 @@ struct TileOp {
               if (oy0 + c < (int64_t)g.H) out[orow * g.ldo + oy0 + c] = cvt_out<TDst>(y[c]);
           }
         }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// One-pass 2-D SAT of 1-byte inputs (u8, i8) into 4-byte outputs with the
+// tile kept packed: a lane holds the 4 input bytes of each of its rows in one
+// register until the output pass, so a wave holds RPT rows in the registers
+// an unpacked tile spends on RPT / 4. With NW = 4 and RPT = 64 a tile is
+// 256 x 256 pixels, four times the pixels of the unpacked 64 x 256 tile for
+// the same carry words per tile (a quarter of the vertical words per pixel).
+//
+// Before the look-backs, the column sums (two 16-bit lanes per register:
+// bytes 0 and 2, bytes 1 and 3) and the row sums (v_dot4 per lane, then a
+// wave scan) come from the packed bytes and are published; the look-backs are
+// TileOp's. After them every row is unpacked, scanned and stored in turn.
+// Tile-local sums are exact 32-bit integers (|sum| <= 256 * 256 * 255 < 2^24,
+// so they are also exact once converted to f32); carries add in the output's
+// arithmetic type, so u8 -> u32 and i8 -> i32 wrap modulo 2^32 as the other
+// operators do.
+// ---------------------------------------------------------------------------
+template <typename TSrc, typename TDst, int SCAN, int NW, int RPT>
+struct PackedTileOp {
+  using Base = TileOp<TSrc, TDst, MODE_SAT, false, false, SCAN, NW, RPT>;
+  using Src = TSrc;
+  using Dst = TDst;
+  using A = typename Base::A;
+  using Smem = typename Base::Smem;
+  static constexpr int TW = Base::TW, TH = Base::TH, NT = Base::NT;
+  static constexpr int64_t VWORDS = Base::VWORDS, HWORDS = Base::HWORDS;
+  static constexpr bool SIGNED = (TSrc)(-1) < (TSrc)0;
+  static_assert(sizeof(TSrc) == 1 && sizeof(A) == 4, "1-byte inputs, 4-byte outputs");
+  static_assert(RPT <= 128, "16-bit column partial sums");
+
+  __device__ static __forceinline__ uint32_t byte_of(uint32_t p, int c) {
+    if constexpr (SIGNED) return (uint32_t)((int32_t)(p << (24 - 8 * c)) >> 24);
+    else return (p >> (8 * c)) & 0xffu;
+  }
+  __device__ static __forceinline__ A to_a(uint32_t x) {
+    if constexpr (sizeof(A) == 4 && (A)0.5f != (A)0) return SIGNED ? (A)(int32_t)x : (A)x;  // f32
+    else return (A)x;
+  }
+
+  // The 4 bytes of columns [col, col+4) of one row; zero outside the image.
+  __device__ static __forceinline__ uint32_t load_edge(const TSrc* __restrict__ row, int64_t col, int64_t W,
+                                                       bool row_ok) {
+    if (!row_ok || col >= W) return 0u;
+    if (W >= 4) {
+      // the last whole chunk [W-4, W) for a chunk crossing the right edge,
+      // shifted so that byte c is column col + c (zeros shift in past W)
+      const int64_t base = col + 3 < W ? col : W - 4;
+      const uint32_t raw = __builtin_bit_cast(uint32_t, *(const typename VecG<TSrc>::type*)(row + base));
+      return raw >> (8 * (int)(col - base));
+    }
+    uint32_t p = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (col + c < W) p |= (uint32_t)(uint8_t)row[col + c] << (8 * c);
+    return p;
+  }
+
+  __device__ static __forceinline__ void run(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const typename Base::Coords k = Base::coords(g, t);
+    const uint32_t I = k.I, J = k.J;
+    const TSrc* __restrict__ in = (const TSrc*)g.in;
+    const bool inner = Base::interior(g, k);
+
+    // ---- load: RPT packed rows per lane ----
+    uint32_t p[RPT];
+    if (inner) {
+      const TSrc* p0 = in + Base::prow_of(k, 0) * g.ldi + k.pcol;
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+        p[r] = __builtin_bit_cast(uint32_t, *(const typename VecG<TSrc>::type*)(p0 + r * g.ldi));
+    } else {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int64_t prow = Base::prow_of(k, r);
+        const bool ok = prow < (int64_t)g.H;
+        p[r] = load_edge(in + (ok ? prow : (int64_t)g.H - 1) * g.ldi, k.pcol, g.W, ok);
+      }
+    }
+
+    // ---- column sums of the wave's rows: bytes 0 / 2 and 1 / 3 summed in
+    //      the two 16-bit halves of a register (i8 biased by 128 to u8) ----
+    {
+      const uint32_t bias = SIGNED ? 0x80808080u : 0u;
+      uint32_t even = 0, odd = 0;
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const uint32_t q = p[r] ^ bias;
+        even += q & 0x00ff00ffu;
+        odd += (q >> 8) & 0x00ff00ffu;
+      }
+      const uint32_t off = SIGNED ? 128u * RPT : 0u;
+      const uint32_t cs[4] = {(even & 0xffffu) - off, (odd & 0xffffu) - off, (even >> 16) - off, (odd >> 16) - off};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sm.ct[w][4 * lane + c] = to_a(cs[c]);
+    }
+    A colsum[Base::CPV];
+    Base::publish_colsums(g, t, I, ep4, sm, colsum);
+
+    // ---- row sums, published as soon as they are known ----
+    {
+      uint32_t mine = 0;  // lane r: sum of this wave's row r (r < 64)
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        uint32_t s4;
+        if constexpr (SIGNED) s4 = (uint32_t)__builtin_amdgcn_sdot4((int)p[r], 0x01010101, 0, false);
+        else s4 = __builtin_amdgcn_udot4(p[r], 0x01010101u, 0u, false);
+        const uint32_t tot = readlane(wave_scan_ks(s4), 63);
+        if (lane == r % 64) mine = tot;
+        if (r % 64 == 63 || r == RPT - 1) {
+          if (lane <= r % 64) {
+            const int i = w * RPT + (r / 64) * 64 + lane;
+            const A rs = to_a(mine);
+            Carry<A>::put(g.hcarry, (int64_t)t * TH + i, rs, ep4 | (J == 0 ? ST_INC : ST_AGG));
+            sm.rs[i] = rs;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    Base::lookbacks(g, t, I, J, ep4, sm, colsum);
+    A colAdd[4];
+    Base::col_add(sm, colAdd);
+    const A corner = sm.corner;
+
+    // ---- output: unpack, scan and store row by row ----
+    TDst* __restrict__ out = (TDst*)g.out;
+    uint32_t colrun[4] = {0u, 0u, 0u, 0u};  // wave-local column prefix (the tile-local SAT)
+    TDst* o0 = out + (k.py0 + w * RPT) * g.ldo + k.pcol;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      uint32_t x[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) x[c] = byte_of(p[r], c);
+      x[1] += x[0];
+      x[2] += x[1];
+      x[3] += x[2];
+      const uint32_t exc = wave_shr1(wave_scan_ks(x[3]));
+      const A rp = sm.rc[w * RPT + r] + corner;
+      A y[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        colrun[c] += x[c] + exc;
+        y[c] = to_a(colrun[c]) + colAdd[c] + rp;
+      }
+      if (inner) {
+        typename VecG<TDst>::type q;
+        q.x = cvt_out<TDst>(y[0]);
+        q.y = cvt_out<TDst>(y[1]);
+        q.z = cvt_out<TDst>(y[2]);
+        q.w = cvt_out<TDst>(y[3]);
+        __builtin_nontemporal_store(q, (typename VecG<TDst>::type*)(o0 + r * g.ldo));
+      } else {
+        const int64_t prow = Base::prow_of(k, r);
+        store4<TDst>(out + prow * g.ldo, k.pcol, g.W, prow < (int64_t)g.H, y);
       }
     }
   }

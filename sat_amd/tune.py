@@ -27,7 +27,12 @@ CFG_NAMES = {0: "4 waves x 16 rows, 64x256 (32x256 for 8-byte outputs), 4 waves 
              1: "8 waves x 16 rows, 128x256 (64x256), 4 waves per SIMD",
              2: "8 waves x 8 rows, 64x256 (32x256), up to 6 waves per SIMD",
              3: "8 waves x 8 rows, 64x256 (32x256), 8 waves per SIMD",
-             4: "4 waves x 24 rows, 96x256 (48x256), 3 waves per SIMD"}
+             4: "4 waves x 24 rows, 96x256 (48x256), 3 waves per SIMD",
+             5: "1-byte inputs into 4-byte outputs: 4 waves x 64 packed rows, 256x256; other pairs as 1"}
+
+# the batch kernel's configurations (0-4; the packed configuration 5 is
+# one-pass only)
+BATCH_CFGS = 5
 
 # (input bytes, output bytes) pairs the table is measured for, with the dtype
 # pair timed for each: u8 -> u32, f32 -> f32, f64 -> f64. Other pairs take
@@ -176,7 +181,7 @@ def _tune_batch(L, sa, out_bytes, n, reps, log):
         pairs.append((xd, yd))
     times = []
     outs = None
-    for cfg in range(L.sat_tile_config_count()):
+    for cfg in range(BATCH_CFGS):
         sa.check(L.sat_batch_tune_set(out_bytes, cfg), "sat_batch_tune_set")
         plan = sa.BatchPlan(pairs)
         plan.run()

@@ -41,7 +41,8 @@ def force(L, in_bytes, out_bytes, w, h, p, cfg):
     assert L.sat_tune_pick(in_bytes, out_bytes, w, h, p) == cfg
 
 
-CFGS = [0, 1, 2, 3, 4]
+CFGS = [0, 1, 2, 3, 4, 5]
+BATCH_CFGS = [0, 1, 2, 3, 4]
 IN_BYTES = {"u8": 1, "i8": 1, "u32": 4, "i32": 4, "f32": 4, "f64": 8}
 
 
@@ -76,7 +77,7 @@ def test_planes_and_reverse_every_config(O, dev, sa, cfg):
         assert_sat_close(O, np.ascontiguousarray(flip), np.ascontiguousarray(r[p, ::-1, ::-1]), "f64")
 
 
-@pytest.mark.parametrize("cfg", CFGS)
+@pytest.mark.parametrize("cfg", BATCH_CFGS)
 def test_batch_every_config(O, dev, sa, cfg):
     say(f"[case] batch cfg={cfg}")
     L = sa.lib()
@@ -103,11 +104,53 @@ def test_tuner_quick(sa, tmp_path):
 
     doc = tune.tune([(512, 2048, 1), (4096, 256, 1)], widths=((1, 4), (4, 4)), batch_out=(4,), batch_images=8,
                     reps=2, log=lambda *m: say(*m))
-    assert len(doc["entries"]) == 4 and all(0 <= e["cfg"] < 5 for e in doc["entries"])
+    assert len(doc["entries"]) == 4 and all(0 <= e["cfg"] < 6 for e in doc["entries"])
     assert {e["in_bytes"] for e in doc["entries"]} == {1, 4}
-    assert doc["batch"][0]["cfg"] in CFGS
+    assert doc["batch"][0]["cfg"] in BATCH_CFGS
     p = tmp_path / "tuned.json"
     import json
 
     p.write_text(json.dumps(doc))
     assert tune.apply_table(p) == 5
+
+
+# The packed 1-byte tiles (configuration 5): partial chunks at the right edge
+# (widths not a multiple of 4, below 4), partial 256-row tiles, rows that
+# start at any byte (pitch and offset), negative i8 values, wraparound.
+PACKED_CASES = [((1, 1), 1, 0), ((2, 3), 3, 1), ((5, 7), 9, 2), ((300, 257), 260, 3), ((257, 1030), 1033, 1),
+                ((1000, 77), 80, 0), ((513, 4099), 4100, 0)]
+
+
+@pytest.mark.parametrize("tin,tout", [("u8", "u32"), ("i8", "i32"), ("u8", "f32"), ("i8", "f32")])
+def test_packed_edges_pitches_offsets(O, dev, sa, tin, tout):
+    say(f"[case] packed tiles {tin}->{tout}")
+    L = sa.lib()
+    for k, ((h, w), pitch, off) in enumerate(PACKED_CASES):
+        force(L, 1, 4, w, h, 1, 5)
+        x = O.gen(KIND[tin], (h, w), 5100 + k)
+        buf = np.zeros(off + h * pitch + 8, dtype=x.dtype)
+        buf[off:off + h * pitch].reshape(h, pitch)[:, :w] = x
+        xd = torch.from_numpy(buf).to(dev)
+        y = torch.empty((h, w), dtype=TD[tout], device=dev)
+        tdt = {"u8": sa.U8, "i8": sa.I8}[tin]
+        odt = {"u32": sa.U32, "i32": sa.I32, "f32": sa.F32}[tout]
+        st = L.sat_integral(3, 0, tdt, odt, xd.data_ptr() + off, pitch, y.data_ptr(), w, w, h, None, 0,
+                            torch.cuda.current_stream().cuda_stream)
+        assert st == 0
+        assert_sat_close(O, x, to_np(y), tout)
+    sa.check_faults()
+
+
+def test_packed_all_255_wraps(dev, sa):
+    say("[case] packed tiles u8 all 255, 4500 x 4500 (wraps modulo 2^32)")
+    L = sa.lib()
+    H = W = 4500
+    force(L, 1, 4, W, H, 1, 5)
+    x = torch.full((H, W), 255, dtype=torch.uint8, device=dev)
+    y = sa.integral(x, out_dtype=torch.uint32).view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    i1 = torch.arange(1, H + 1, dtype=torch.int64, device=dev)[:, None]
+    j1 = torch.arange(1, W + 1, dtype=torch.int64, device=dev)[None, :]
+    assert torch.equal(y, (255 * i1 * j1) & 0xFFFFFFFF)
+    xs = torch.full((H, W), -128, dtype=torch.int8, device=dev)
+    ys = sa.integral(xs, out_dtype=torch.int32).to(torch.int64) & 0xFFFFFFFF
+    assert torch.equal(ys, (-128 * i1 * j1) & 0xFFFFFFFF)
