@@ -14,8 +14,10 @@ __global__ void __launch_bounds__(256) clear_kernel(uint64_t* p, uint64_t n) {
 // Streaming copy, the bandwidth reference the SAT kernels are measured
 // against: each thread moves COPY_UNROLL 16-byte chunks, every load of a wave
 // covering 1 KiB of consecutive bytes, all loads issued before the first
-// (nontemporal) store. Chunks past `n16` are skipped; `tail` bytes after the
-// last whole chunk are copied by the first workgroup.
+// store; loads and stores are nontemporal (the fastest of the copy shapes
+// measured on buffers the MALL does not hold). Chunks past `n16` are
+// skipped; `tail` bytes after the last whole chunk are copied by the first
+// workgroup.
 constexpr int COPY_UNROLL = 4;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) stream_copy_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
@@ -26,7 +28,7 @@ __global__ void __launch_bounds__(256) stream_copy_kernel(const u32x4* __restric
 #pragma unroll
   for (int u = 0; u < COPY_UNROLL; ++u) {
     const uint64_t i = base + (uint64_t)u * 256;
-    if (i < n16) v[u] = src[i];
+    if (i < n16) v[u] = __builtin_nontemporal_load(src + i);
   }
 #pragma unroll
   for (int u = 0; u < COPY_UNROLL; ++u) {

@@ -142,8 +142,15 @@ struct TileOp {
       typename VecG<TSrc>::type q[RPT];
       const TSrc* p0 = in + prow_of(k, 0) * g.ldi + k.pcol;
       const int64_t step = REV ? -g.ldi : g.ldi;
+      // 4-byte inputs are read nontemporally: the input is streamed once, and
+      // keeping it out of the caches measured 1-3 % faster (f32 8192^2,
+      // 65536 x 256; neutral or slower for 1- and 8-byte inputs)
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) q[r] = *(const typename VecG<TSrc>::type*)(p0 + r * step);
+      for (int r = 0; r < RPT; ++r) {
+        const typename VecG<TSrc>::type* a = (const typename VecG<TSrc>::type*)(p0 + r * step);
+        if constexpr (sizeof(TSrc) == 4) q[r] = __builtin_nontemporal_load(a);
+        else q[r] = *a;
+      }
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const A x[4] = {cvt_in<TSrc, TDst, A>(q[r].x), cvt_in<TSrc, TDst, A>(q[r].y),
