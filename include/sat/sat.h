@@ -175,6 +175,21 @@ int sat_integral_planes(int alg, int scan, int tin, int tout, const void* in, in
                         uint32_t width, uint32_t height, uint32_t planes, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* One-pass SAT (algorithm 3) that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device or pinned memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Its carry workspace is a device array of
+ * sat_integral_noalloc_max_bytes() (32 MiB) bytes that the library holds on
+ * every device it runs on until the process ends; shapes whose workspace
+ * (sat_integral_workspace_bytes(3, ...)) exceeds it are refused with
+ * SAT_ERR_WORKSPACE (about 200 million pixels fit, f32 14000 x 14000). Calls on one
+ * device share that array: calls that can overlap on one device (other
+ * streams, other host threads) race and must not be issued. Arguments and
+ * results as sat_integral with alg = 3. */
+size_t sat_integral_noalloc_max_bytes(void);
+int sat_integral_noalloc(int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                         uint32_t width, uint32_t height, void* stream);
+
 /* OpenCV-style layout (cv::integral): `out` is (height+1) x (width+1) with
  * row 0 and column 0 zero and out[y+1][x+1] = sum_{k<=y, l<=x} in[k][l];
  * out_pitch >= width + 1 elements. Same algorithms, pairs and workspace as

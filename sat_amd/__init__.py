@@ -67,6 +67,8 @@ SIGNATURES = {
     "sat_box_scatter": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp]),
     "sat_box_grad_workspace_bytes": (_sz, [_i, _u32, _u32, _u32, _u32]),
     "sat_box_grad": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp, _sz, _vp]),
+    "sat_integral_noalloc_max_bytes": (_sz, []),
+    "sat_integral_noalloc": (_i, [_i, _i, _i, _vp, _i64, _vp, _i64, _u32, _u32, _vp]),
     "sat_integral_bordered": (_i, [_i, _i, _i, _i, _vp, _i64, _vp, _i64, _u32, _u32, _vp, _sz, _vp]),
     "sat_integral_bordered_planes": (_i, [_i, _i, _i, _i, _vp, _i64, _i64, _vp, _i64, _i64, _u32, _u32, _u32, _vp,
                                           _sz, _vp]),

@@ -245,11 +245,12 @@ static std::mutex g_hf_mu;
 static uint32_t* g_hf_host = nullptr;  // host view
 static uint32_t* g_hf_dev[MAX_DEVICES] = {};
 
-uint32_t* host_fault_word() {
+uint32_t* host_fault_word(bool may_alloc) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
   std::lock_guard<std::mutex> lk(g_hf_mu);
   if (!g_hf_host) {
+    if (!may_alloc) return nullptr;
     void* p = nullptr;
     if (hipHostMalloc(&p, MAX_DEVICES * sizeof(uint32_t),
                       hipHostMallocMapped | hipHostMallocPortable | hipHostMallocCoherent) != hipSuccess)
@@ -600,6 +601,35 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
     it.tile_base = base;
     base += nTx * nTy;
   }
+  return SAT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace of sat_integral_noalloc: a file-scope device array, so that the
+// entry point allocates nothing. Its device address on each device is looked
+// up once (hipGetSymbolAddress, with that device current) on the entry
+// point's first call there; host code uses only that address. Every call on
+// a device shares the device's array, which keeps the last call's contents.
+// ---------------------------------------------------------------------------
+constexpr size_t SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The current device's address of `scratch`; on the first call for a device
+// its header (launch state and fault word) is zeroed on `st`, a fill kernel.
+static sat_status scratch_on_device(hipStream_t st, void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return SAT_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || ptr == nullptr) return SAT_ERR_HIP;
+    const sat_status s = launch_clear(ptr, satdev::WS_HEADER_BYTES, st);
+    if (s != SAT_OK) return s;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
   return SAT_OK;
 }
 
@@ -1004,6 +1034,26 @@ int sat_box_grad(int dtype, const int32_t* boxes, uint32_t nbox, const void* gra
   if ((s = carry_epoch(c, st, &ep)) != SAT_OK) return s;
   return launch_box_grad(dtype, boxes, nbox, grad_out, grad_x, pitch, plane_stride, width, height, planes, c.ws,
                          c.bytes, bins, ep, st);
+}
+
+size_t sat_integral_noalloc_max_bytes(void) { return SCRATCH_BYTES; }
+
+int sat_integral_noalloc(int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out, int64_t out_pitch,
+                         uint32_t width, uint32_t height, void* stream) {
+  if (!scan_ok(scan)) return SAT_ERR_INVALID;
+  if (!pair_ok(tin, tout)) return SAT_ERR_DTYPE;
+  if (width == 0 || height == 0) return SAT_OK;
+  sat_status c = check_2d(in, out, in_pitch, out_pitch, width, height, false);
+  if (c != SAT_OK) return c;
+  SatPlan P;
+  c = plan_integral(SAT_ALG_SINGLE_PASS, scan, tin, tout, in, in_pitch, 0, out, out_pitch, 0, width, height, 1, &P);
+  if (c != SAT_OK) return c;
+  if (P.total > SCRATCH_BYTES) return SAT_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  void* ws = nullptr;
+  if ((c = scratch_on_device(st, &ws)) != SAT_OK) return c;
+  P.p1.may_alloc = false;
+  return run_integral(P, ws, SCRATCH_BYTES, st);
 }
 
 }  // extern "C"

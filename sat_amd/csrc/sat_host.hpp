@@ -48,6 +48,7 @@ struct Launch {
   uint32_t width, height, planes;
   uint32_t epoch;  // launch epoch; 0 = read the workspace header's device epoch
   int cfg = 0;     // tile configuration (one-pass families)
+  bool may_alloc = true;  // false: the launch must not allocate (no first-use pinned host flag)
 };
 
 size_t carry_bytes(const Launch& L);
@@ -92,8 +93,9 @@ sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream)
 
 // Device address of the current device's sticky give-up flag in pinned host
 // memory (the host reads it without synchronising); null if it could not be
-// mapped, which only loses the host-visible copy of a give-up.
-uint32_t* host_fault_word();
+// mapped, which only loses the host-visible copy of a give-up. The pinned
+// memory is allocated on first use; with may_alloc false, null until then.
+uint32_t* host_fault_word(bool may_alloc = true);
 
 // Misc.
 int device_cu_count();

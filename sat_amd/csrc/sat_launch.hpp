@@ -93,7 +93,7 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   g.vcarry = carries;
   g.hcarry = carries + vw;
   g.fault.ws = &((satdev::WsHeader*)ws)->fault;
-  g.fault.host = host_fault_word();
+  g.fault.host = host_fault_word(L.may_alloc);
   satdev::TileOrder order;
   order.nx = (uint32_t)device_xcd_count();
   order.run = (order.nx > 1 && nTx % order.nx == 0) ? (uint32_t)(nTx / order.nx) : 0u;
