@@ -12,7 +12,9 @@ INT_OUT = {"u32", "i32"}
 # reference's own f32 CPU left fold has an error bound that grows with the
 # image size instead; see test_float_error_vs_cpu_reference.
 F32_K = 32.0
-F64_K = 32.0
+# f64: the largest error measured on the 4096^2 N(0,1) workload is 1.70 u *
+# SAT(|x|) for every algorithm (bench.py reports it); the bound is about twice that
+F64_K = 4.0
 
 
 def sat_abs_f64(O, x: np.ndarray) -> np.ndarray:
