@@ -129,8 +129,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--profile", action="store_true",
-                    help="profiling run of one workload (--profile-workload): --steps rotating calls, then "
-                         "3 plain copies of 256 MiB (the counter calibration), no checks and no CPU baseline")
+                    help="profiling run of one workload (--profile-workload): --steps + 1 rotating calls, then "
+                         "3 plain copies of 1 GiB (the counter calibration), no checks and no CPU baseline")
     ap.add_argument("--profile-workload", default="f32_8192", choices=PROFILE_WORKLOADS)
     ap.add_argument("--profile-alg", type=int, default=3, choices=[0, 1, 2, 3],
                     help="algorithm of a --profile run of a single-image workload (0-2: the reference's "
@@ -641,7 +641,7 @@ def profile_run(sat_amd, dev, args):
             b.backward(g.view(b.shape))
         what = f"{args.steps + 1} box_filter forward + backward"
     torch.cuda.synchronize()
-    # counter calibration: plain copies of a known size
+    # counter calibration: plain copies of a known size (1 GiB each way)
     c0 = torch.empty(64 << 22, dtype=torch.float32, device=dev)
     c1 = torch.empty_like(c0)
     for _ in range(3):
