@@ -340,6 +340,17 @@ def batched_config(O, sat_amd, dev, reps=10):
 
     ms_b = time_launches(lambda k: plan.run(), reps, 2)
     plan.check_faults()
+    # the same queue run by the library's other launch form (persistent
+    # workgroups drawing items from per-XCD ticket counters, or one workgroup
+    # per item)
+    lib = sat_amd.lib()
+    mode = lib.sat_batch_persistent_get()
+    lib.sat_batch_persistent_set(1 - mode)
+    plan_alt = sat_amd.BatchPlan(pairs)
+    lib.sat_batch_persistent_set(mode)
+    ms_alt = time_launches(lambda k: plan_alt.run(), reps, 2)
+    plan_alt.check_faults()
+    del plan_alt
     ms_p = time_launches(per_image, max(2, reps // 2), 1)
     check_faults(sat_amd, "per-image launches")
     # the per-image path without Python: the same 512 launches captured once
@@ -369,6 +380,8 @@ def batched_config(O, sat_amd, dev, reps=10):
             raise AssertionError(f"graph replay of image {i} differs from the batched result")
     del ref
     res = {"images": n, "algorithmic_bytes": algo, "batched_ms": round(ms_b, 4),
+           "batched_launch_form": "persistent" if mode else "one workgroup per item",
+           ("batched_one_workgroup_per_item_ms" if mode else "batched_persistent_ms"): round(ms_alt, 4),
            "per_image_launch_ms_python_loop": round(ms_p, 4),
            "per_image_launch_ms_captured_graph": round(ms_g, 4),
            "speedup_vs_per_image_python_loop": round(ms_p / ms_b, 2),

@@ -238,10 +238,17 @@ size_t sat_batch_workspace_bytes(const sat_image* images, int count);
 /* Validates the batch, builds the work queue on the host and copies it into
  * the workspace (blocking copy; call outside graph capture). */
 int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_t workspace_bytes, void* stream);
-/* Runs the prepared queue: one workgroup per (image, tile) item in a single
- * launch (preceded by a one-workgroup epoch bump). Allocates nothing, copies
- * nothing: may be captured in a hipGraph and replayed. */
+/* Runs the prepared queue in a single launch (preceded by a one-workgroup
+ * epoch bump). Allocates nothing, copies nothing: may be captured in a
+ * hipGraph and replayed. */
 int sat_batch_run(void* workspace, void* stream);
+/* Form of the launch for plans prepared from now on: 0 (default) one
+ * workgroup per queue item, which the hardware dispatches in queue order; 1
+ * a persistent kernel, as many workgroups as are resident, each drawing the
+ * next (image, tile) item of its XCD's queue from a ticket counter. Results
+ * are the same; the default measured faster on MI355X. */
+int sat_batch_persistent_set(int on);
+int sat_batch_persistent_get(void);
 
 /* ---------------------------------------------------------------------------
  * Tile configurations and the autotuner's table. No reference counterpart:

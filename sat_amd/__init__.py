@@ -63,6 +63,8 @@ SIGNATURES = {
     "sat_batch_workspace_bytes": (_sz, [_vp, _i]),
     "sat_batch_prepare": (_i, [_vp, _i, _vp, _sz, _vp]),
     "sat_batch_run": (_i, [_vp, _vp]),
+    "sat_batch_persistent_set": (_i, [_i]),
+    "sat_batch_persistent_get": (_i, []),
     "sat_box_sums": (_i, [_i, _vp, _i64, _i64, _u32, _u32, _u32, _vp, _u32, _vp, _vp]),
     "sat_box_scatter": (_i, [_i, _vp, _u32, _vp, _vp, _i64, _i64, _u32, _u32, _u32, _vp]),
     "sat_box_grad_workspace_bytes": (_sz, [_i, _u32, _u32, _u32, _u32]),

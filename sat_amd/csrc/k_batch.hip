@@ -28,8 +28,42 @@ __device__ __forceinline__ void run_kind(const ImgArgs& g, uint32_t lt, uint32_t
   Op::run(g, lt, ep4, *reinterpret_cast<typename Op::Smem*>(smem));
 }
 
-// One workgroup per queue item: item b = blockIdx.x is tile queue[2b+1] of
-// image queue[2b] (an empty item returns at once). The launch epoch is the
+// Runs queue item q: tile queue[2q+1] of image queue[2q] (an empty item
+// returns at once). `sm` is the reference operator's shared memory, which
+// every operator of the launch's output width fits in.
+template <int OUTB, int CFG>
+__device__ __forceinline__ void run_item(const sathost::BatchItem* __restrict__ items,
+                                         const uint32_t* __restrict__ queue, uint32_t q, uint32_t ep4, void* sm) {
+  const uint32_t img = __builtin_amdgcn_readfirstlane(queue[2 * q]);
+  if (img == 0xffffffffu) return;
+  const uint32_t lt = __builtin_amdgcn_readfirstlane(queue[2 * q + 1]);
+  const sathost::BatchItem& it = items[img];
+  const ImgArgs g = it.g;
+  const uint32_t kind = __builtin_amdgcn_readfirstlane(it.kind);
+  if constexpr (OUTB == 4) {
+    switch (kind) {
+      case K_U8_U32: run_kind<4, CFG, uint8_t, uint32_t>(g, lt, ep4, sm); break;
+      case K_U8_F32: run_kind<4, CFG, uint8_t, float>(g, lt, ep4, sm); break;
+      case K_I8_I32: run_kind<4, CFG, int8_t, int32_t>(g, lt, ep4, sm); break;
+      case K_I8_F32: run_kind<4, CFG, int8_t, float>(g, lt, ep4, sm); break;
+      case K_I32_I32: run_kind<4, CFG, int32_t, int32_t>(g, lt, ep4, sm); break;
+      case K_F32_F32: run_kind<4, CFG, float, float>(g, lt, ep4, sm); break;
+      case K_U32_U32: run_kind<4, CFG, uint32_t, uint32_t>(g, lt, ep4, sm); break;
+      default: break;
+    }
+  } else {
+    switch (kind) {
+      case K_U8_F64: run_kind<8, CFG, uint8_t, double>(g, lt, ep4, sm); break;
+      case K_I8_F64: run_kind<8, CFG, int8_t, double>(g, lt, ep4, sm); break;
+      case K_I32_F64: run_kind<8, CFG, int32_t, double>(g, lt, ep4, sm); break;
+      case K_F32_F64: run_kind<8, CFG, float, double>(g, lt, ep4, sm); break;
+      case K_F64_F64: run_kind<8, CFG, double, double>(g, lt, ep4, sm); break;
+      default: break;
+    }
+  }
+}
+
+// One workgroup per queue item: item b = blockIdx.x. The launch epoch is the
 // workspace's device epoch, advanced by bump_epoch_kernel before every
 // launch, so the same launch can be replayed from a captured graph.
 template <int OUTB, int CFG>
@@ -38,36 +72,45 @@ __global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW))
     batch_kernel(const WsHeader* hdr, const sathost::BatchItem* __restrict__ items,
                  const uint32_t* __restrict__ queue) {
   using Cfg = BatchCfg<OUTB, CFG>;
-  using Ref = typename Cfg::Ref;
-  using RefOp = TileOp<Ref, Ref, MODE_SAT, false, false, 0, Cfg::NW, Cfg::RPT>;
+  using RefOp = TileOp<typename Cfg::Ref, typename Cfg::Ref, MODE_SAT, false, false, 0, Cfg::NW, Cfg::RPT>;
   __shared__ typename RefOp::Smem sm;
+  run_item<OUTB, CFG>(items, queue, blockIdx.x, launch_ep4(0, hdr), &sm);
+}
+
+// Persistent form: nx * G workgroups, each serving the XCD x = blockIdx.x % nx
+// it is dealt to. It draws tickets k = 0, 1, ... from that XCD's counter and
+// runs queue item k * nx + x (the XCD's k-th item) until k reaches the list
+// length L; the next ticket is drawn while the current item runs. Tickets are
+// drawn in increasing order and every item waits only on earlier items of its
+// own list, which workgroups that are running have drawn: the walk progresses
+// whatever the number of resident workgroups. The counters are zeroed by
+// bump_epoch_kernel before every launch.
+template <int OUTB, int CFG>
+__global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW))
+    __attribute__((amdgpu_waves_per_eu(BatchCfg<OUTB, CFG>::WPE)))
+    batch_persistent_kernel(const WsHeader* hdr, const sathost::BatchItem* __restrict__ items,
+                            const uint32_t* __restrict__ queue, uint32_t* tickets, uint32_t L, uint32_t nx) {
+  using Cfg = BatchCfg<OUTB, CFG>;
+  using RefOp = TileOp<typename Cfg::Ref, typename Cfg::Ref, MODE_SAT, false, false, 0, Cfg::NW, Cfg::RPT>;
+  __shared__ typename RefOp::Smem sm;
+  __shared__ uint32_t s_k[2];
   const uint32_t ep4 = launch_ep4(0, hdr);
-  const uint32_t img = __builtin_amdgcn_readfirstlane(queue[2 * blockIdx.x]);
-  if (img == 0xffffffffu) return;
-  const uint32_t lt = __builtin_amdgcn_readfirstlane(queue[2 * blockIdx.x + 1]);
-  const sathost::BatchItem& it = items[img];
-  const ImgArgs g = it.g;
-  const uint32_t kind = __builtin_amdgcn_readfirstlane(it.kind);
-  if constexpr (OUTB == 4) {
-    switch (kind) {
-      case K_U8_U32: run_kind<4, CFG, uint8_t, uint32_t>(g, lt, ep4, &sm); break;
-      case K_U8_F32: run_kind<4, CFG, uint8_t, float>(g, lt, ep4, &sm); break;
-      case K_I8_I32: run_kind<4, CFG, int8_t, int32_t>(g, lt, ep4, &sm); break;
-      case K_I8_F32: run_kind<4, CFG, int8_t, float>(g, lt, ep4, &sm); break;
-      case K_I32_I32: run_kind<4, CFG, int32_t, int32_t>(g, lt, ep4, &sm); break;
-      case K_F32_F32: run_kind<4, CFG, float, float>(g, lt, ep4, &sm); break;
-      case K_U32_U32: run_kind<4, CFG, uint32_t, uint32_t>(g, lt, ep4, &sm); break;
-      default: break;
-    }
-  } else {
-    switch (kind) {
-      case K_U8_F64: run_kind<8, CFG, uint8_t, double>(g, lt, ep4, &sm); break;
-      case K_I8_F64: run_kind<8, CFG, int8_t, double>(g, lt, ep4, &sm); break;
-      case K_I32_F64: run_kind<8, CFG, int32_t, double>(g, lt, ep4, &sm); break;
-      case K_F32_F64: run_kind<8, CFG, float, double>(g, lt, ep4, &sm); break;
-      case K_F64_F64: run_kind<8, CFG, double, double>(g, lt, ep4, &sm); break;
-      default: break;
-    }
+  const uint32_t x = blockIdx.x % nx;
+  uint32_t* tk = tickets + BATCH_TICKET_STRIDE * x;
+  if (threadIdx.x == 0) s_k[0] = __hip_atomic_fetch_add(tk, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  uint32_t k = s_k[0];
+  int cur = 0;
+  while (k < L) {
+    uint32_t next = 0;
+    if (threadIdx.x == 0) next = __hip_atomic_fetch_add(tk, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    run_item<OUTB, CFG>(items, queue, k * nx + x, ep4, &sm);
+    // every wave is done with the item's shared memory before the next item
+    // writes it; the next ticket becomes visible to all waves
+    if (threadIdx.x == 0) s_k[cur ^ 1] = next;
+    __syncthreads();
+    cur ^= 1;
+    k = s_k[cur];
   }
 }
 
@@ -109,26 +152,48 @@ Geometry batch_geometry(size_t out_size, int cfg) {
 
 template <int OUTB, int CFG>
 static void launch_batch_cfg(const BatchItem* items, const uint32_t* queue, uint32_t blocks, void* ws,
-                             hipStream_t stream) {
-  hipLaunchKernelGGL((satdev::batch_kernel<OUTB, CFG>), dim3(blocks), dim3(64 * satdev::BatchCfg<OUTB, CFG>::NW),
-                     0, stream, (const satdev::WsHeader*)ws, items, queue);
+                             const BatchPersist& pers, hipStream_t stream) {
+  constexpr int threads = 64 * satdev::BatchCfg<OUTB, CFG>::NW;
+  if (pers.tickets)
+    hipLaunchKernelGGL((satdev::batch_persistent_kernel<OUTB, CFG>), dim3(pers.grid), dim3(threads), 0, stream,
+                       (const satdev::WsHeader*)ws, items, queue, pers.tickets, pers.items_per_xcd, pers.nx);
+  else
+    hipLaunchKernelGGL((satdev::batch_kernel<OUTB, CFG>), dim3(blocks), dim3(threads), 0, stream,
+                       (const satdev::WsHeader*)ws, items, queue);
+}
+
+template <int OUTB, int CFG>
+static int resident_cfg() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, satdev::batch_persistent_kernel<OUTB, CFG>,
+                                                   64 * satdev::BatchCfg<OUTB, CFG>::NW, 0) != hipSuccess)
+    return 0;
+  return n;
+}
+
+int batch_resident_per_cu(int out_size, int cfg) {
+  if (out_size == 8)
+    return cfg == 1 ? resident_cfg<8, 1>() : cfg == 2 ? resident_cfg<8, 2>() : cfg == 3 ? resident_cfg<8, 3>()
+         : cfg == 4 ? resident_cfg<8, 4>() : resident_cfg<8, 0>();
+  return cfg == 1 ? resident_cfg<4, 1>() : cfg == 2 ? resident_cfg<4, 2>() : cfg == 3 ? resident_cfg<4, 3>()
+       : cfg == 4 ? resident_cfg<4, 4>() : resident_cfg<4, 0>();
 }
 
 sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* queue, uint32_t blocks,
-                        void* ws, hipStream_t stream) {
+                        void* ws, const BatchPersist& pers, hipStream_t stream) {
   if (blocks == 0) return SAT_OK;
   if (out_size == 8) {
-    if (cfg == 1) launch_batch_cfg<8, 1>(items, queue, blocks, ws, stream);
-    else if (cfg == 2) launch_batch_cfg<8, 2>(items, queue, blocks, ws, stream);
-    else if (cfg == 3) launch_batch_cfg<8, 3>(items, queue, blocks, ws, stream);
-    else if (cfg == 4) launch_batch_cfg<8, 4>(items, queue, blocks, ws, stream);
-    else launch_batch_cfg<8, 0>(items, queue, blocks, ws, stream);
+    if (cfg == 1) launch_batch_cfg<8, 1>(items, queue, blocks, ws, pers, stream);
+    else if (cfg == 2) launch_batch_cfg<8, 2>(items, queue, blocks, ws, pers, stream);
+    else if (cfg == 3) launch_batch_cfg<8, 3>(items, queue, blocks, ws, pers, stream);
+    else if (cfg == 4) launch_batch_cfg<8, 4>(items, queue, blocks, ws, pers, stream);
+    else launch_batch_cfg<8, 0>(items, queue, blocks, ws, pers, stream);
   } else {
-    if (cfg == 1) launch_batch_cfg<4, 1>(items, queue, blocks, ws, stream);
-    else if (cfg == 2) launch_batch_cfg<4, 2>(items, queue, blocks, ws, stream);
-    else if (cfg == 3) launch_batch_cfg<4, 3>(items, queue, blocks, ws, stream);
-    else if (cfg == 4) launch_batch_cfg<4, 4>(items, queue, blocks, ws, stream);
-    else launch_batch_cfg<4, 0>(items, queue, blocks, ws, stream);
+    if (cfg == 1) launch_batch_cfg<4, 1>(items, queue, blocks, ws, pers, stream);
+    else if (cfg == 2) launch_batch_cfg<4, 2>(items, queue, blocks, ws, pers, stream);
+    else if (cfg == 3) launch_batch_cfg<4, 3>(items, queue, blocks, ws, pers, stream);
+    else if (cfg == 4) launch_batch_cfg<4, 4>(items, queue, blocks, ws, pers, stream);
+    else launch_batch_cfg<4, 0>(items, queue, blocks, ws, pers, stream);
   }
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }

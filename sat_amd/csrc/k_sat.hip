@@ -39,16 +39,23 @@ __global__ void __launch_bounds__(256) stream_copy_kernel(const u32x4* __restric
 }
 
 // Advances the device epoch of a workspace for the launch that follows on the
-// same stream; on wrap-around it clears `words` carry words first.
-__global__ void __launch_bounds__(256) bump_epoch_kernel(WsHeader* hdr, uint64_t* carries, uint64_t words) {
-  __shared__ uint32_t s_next;
+// same stream. A header below the device range (a fresh or cleared area, or
+// one only host-epoch launches used) starts it at EPOCH_DEVICE_BASE with no
+// clear: such an area holds no word tagged with a device epoch. When the
+// range wraps, `words` carry words are cleared first, so no word of an
+// earlier pass through the range can pass for a new launch's.
+__global__ void __launch_bounds__(256) bump_epoch_kernel(WsHeader* hdr, uint64_t* carries, uint64_t words,
+                                                         uint32_t* tickets, uint32_t ntickets) {
+  if (threadIdx.x < ntickets) tickets[threadIdx.x * BATCH_TICKET_STRIDE] = 0;
+  __shared__ uint32_t s_next, s_wrap;
   if (threadIdx.x == 0) {
     const uint32_t e = hdr->epoch;
+    s_wrap = e >= EPOCH_MAX;
     s_next = (e < EPOCH_DEVICE_BASE || e >= EPOCH_MAX) ? EPOCH_DEVICE_BASE : e + 1;
   }
   __syncthreads();
   const uint32_t next = s_next;
-  if (next == EPOCH_DEVICE_BASE)
+  if (s_wrap)
     for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) carries[i] = 0;
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(&hdr->epoch, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -87,9 +94,11 @@ sat_status launch_border(size_t out_size, void* out, int64_t pitch, int64_t plan
 
 using namespace satdev;
 
-sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream) {
+sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream, uint32_t* tickets,
+                             uint32_t ntickets) {
+  if (ntickets > 256) return SAT_ERR_INVALID;
   hipLaunchKernelGGL(bump_epoch_kernel, dim3(1), dim3(256), 0, stream, (WsHeader*)ws,
-                     (uint64_t*)((char*)ws + WS_HEADER_BYTES), carry_words);
+                     (uint64_t*)((char*)ws + WS_HEADER_BYTES), carry_words, tickets, ntickets);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;
 }
 

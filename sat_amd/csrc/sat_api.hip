@@ -500,10 +500,18 @@ struct BatchPlanInfo {
   int out_size = 0;
   int cfg = 0;
   uint32_t total_tiles = 0;
-  uint32_t blocks = 0;  // workgroups of the launch (tiles + empty items)
+  uint32_t blocks = 0;  // queue items (tiles + empty items), items_per_xcd * nx
+  uint32_t nx = 1;
   uint64_t carry_words = 0;
-  size_t items_off = 0, queue_off = 0, total = 0;
+  size_t items_off = 0, queue_off = 0, tickets_off = 0, total = 0;
+  bool persistent = false;
+  uint32_t grid = 0;  // workgroups of the persistent form
 };
+// Batch plans prepared from now on use the persistent kernel (1) or one
+// workgroup per queue item (0, the default: the hardware's workgroup
+// dispatch refills freed slots faster than ticket counters hand out items;
+// 512 uneven images: 0.95 ms against 1.18 ms persistent).
+static std::atomic<int> g_batch_persistent{0};
 static std::mutex g_batch_mu;
 static std::map<uintptr_t, BatchPlanInfo> g_batch;
 
@@ -561,10 +569,12 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
   info->cfg = cfg;
   info->total_tiles = (uint32_t)tiles;
   info->blocks = (uint32_t)blocks;
+  info->nx = (uint32_t)nx;
   info->carry_words = tiles * (uint64_t)(g.vwords + g.hwords);
   info->items_off = (size_t)round_up(satdev::WS_HEADER_BYTES + (int64_t)info->carry_words * 8, 256);
   info->queue_off = (size_t)round_up((int64_t)(info->items_off + sizeof(BatchItem) * (size_t)count), 256);
-  info->total = info->queue_off + 2 * sizeof(uint32_t) * (size_t)blocks;
+  info->tickets_off = (size_t)round_up((int64_t)(info->queue_off + 2 * sizeof(uint32_t) * (size_t)blocks), 256);
+  info->total = info->tickets_off + (size_t)nx * satdev::BATCH_TICKET_STRIDE * sizeof(uint32_t);
   if (!items) return SAT_OK;
   items->resize(count);
   queue->assign(2 * blocks, QUEUE_EMPTY);
@@ -949,6 +959,15 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
                      hipMemcpyHostToDevice, st) != hipSuccess)
     return SAT_ERR_HIP;
   if (hipStreamSynchronize(st) != hipSuccess) return SAT_ERR_HIP;
+  info.persistent = g_batch_persistent.load(std::memory_order_relaxed) != 0 && info.nx <= 256;
+  if (info.persistent) {
+    // as many workgroups per XCD as are resident at once (more would only
+    // wait), never more than the XCD's queue items
+    const int per_cu = batch_resident_per_cu(info.out_size, info.cfg);
+    const uint64_t per_xcd = (uint64_t)std::max(1, per_cu) * std::max(1, device_cu_count() / (int)info.nx);
+    info.grid = (uint32_t)(std::min<uint64_t>(per_xcd, info.blocks / info.nx) * info.nx);
+    if (info.grid == 0) info.persistent = false;
+  }
   std::lock_guard<std::mutex> lk(g_batch_mu);
   g_batch[(uintptr_t)workspace] = info;
   return SAT_OK;
@@ -965,11 +984,25 @@ int sat_batch_run(void* workspace, void* stream) {
   }
   if (info.total_tiles == 0) return SAT_OK;
   hipStream_t st = (hipStream_t)stream;
-  const sat_status s = launch_bump_epoch(workspace, info.carry_words, st);
+  BatchPersist pers;
+  if (info.persistent) {
+    pers.tickets = (uint32_t*)((char*)workspace + info.tickets_off);
+    pers.grid = info.grid;
+    pers.items_per_xcd = info.blocks / info.nx;
+    pers.nx = info.nx;
+  }
+  const sat_status s = launch_bump_epoch(workspace, info.carry_words, st, pers.tickets, pers.tickets ? info.nx : 0);
   if (s != SAT_OK) return s;
   return launch_batch(info.out_size, info.cfg, (const BatchItem*)((char*)workspace + info.items_off),
-                      (const uint32_t*)((char*)workspace + info.queue_off), info.blocks, workspace, st);
+                      (const uint32_t*)((char*)workspace + info.queue_off), info.blocks, workspace, pers, st);
 }
+
+int sat_batch_persistent_set(int on) {
+  g_batch_persistent.store(on ? 1 : 0, std::memory_order_relaxed);
+  return SAT_OK;
+}
+
+int sat_batch_persistent_get(void) { return g_batch_persistent.load(std::memory_order_relaxed); }
 
 int sat_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width, uint32_t height,
                  uint32_t planes, const int32_t* boxes, uint32_t nbox, void* out, void* stream) {

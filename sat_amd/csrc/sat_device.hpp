@@ -485,7 +485,10 @@ struct ImgArgs {
 //   - launches that must not depend on host state (graph-captured calls and
 //     the batched queue) first run bump_epoch_kernel, which advances the
 //     header's device epoch in [EPOCH_DEVICE_BASE, EPOCH_MAX] and clears the
-//     carry area itself when that range wraps; their workgroups read it;
+//     carry area itself when that range wraps; their workgroups read it.
+//     The header is only ever lowered together with a clear of the carry
+//     area, so while it is below EPOCH_DEVICE_BASE no carry word holds a
+//     device epoch;
 //   - calls on a caller-provided area clear it first and count from 1.
 // ---------------------------------------------------------------------------
 struct WsHeader {
@@ -495,6 +498,9 @@ struct WsHeader {
   uint32_t pad1[31];
 };
 constexpr uint32_t EPOCH_MAX = (1u << 30) - 1;
+// Ticket counters of the persistent batch kernel: one per XCD, each on its
+// own 128-byte line (stride in 32-bit words).
+constexpr uint32_t BATCH_TICKET_STRIDE = 32;
 constexpr uint32_t EPOCH_DEVICE_BASE = 1u << 29;
 constexpr int64_t WS_HEADER_BYTES = 256;
 static_assert(sizeof(WsHeader) <= WS_HEADER_BYTES, "header size");

@@ -63,10 +63,21 @@ struct BatchItem {
   uint32_t tile_base;  // first global tile of this image
 };
 int pair_id(int tin, int tout);  // -1 if unsupported
-// queue: (image, tile) pairs, one per workgroup (image 0xffffffff: empty)
+// The persistent form of the batch launch (tickets null: one workgroup per
+// queue item instead): grid workgroups draw tickets from nx counters.
+struct BatchPersist {
+  uint32_t* tickets = nullptr;
+  uint32_t grid = 0;
+  uint32_t items_per_xcd = 0;
+  uint32_t nx = 1;
+};
+// queue: (image, tile) pairs, blocks = items_per_xcd * nx of them, item k of
+// XCD x at k * nx + x (image 0xffffffff: empty)
 sat_status launch_batch(int out_size, int cfg, const BatchItem* items, const uint32_t* queue, uint32_t blocks,
-                        void* ws, hipStream_t stream);
+                        void* ws, const BatchPersist& pers, hipStream_t stream);
 Geometry batch_geometry(size_t out_size, int cfg);
+// resident workgroups per CU of the persistent batch kernel (0 if unknown)
+int batch_resident_per_cu(int out_size, int cfg);
 
 // Box filter (k_box.hip).
 sat_status launch_box_sums(int dtype, const void* sat, int64_t pitch, int64_t plane_stride, uint32_t width,
@@ -88,8 +99,10 @@ sat_status launch_stream_copy(const void* src, void* dst, size_t bytes, hipStrea
 // Zero row 0 and column 0 of `planes` (H+1) x (W+1) tables (k_sat.hip).
 sat_status launch_border(size_t out_size, void* out, int64_t pitch, int64_t plane, uint32_t width, uint32_t height,
                          uint32_t planes, hipStream_t stream);
-// Advance the header's device epoch (clearing `carry_words` on wrap), k_sat.hip.
-sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream);
+// Advance the header's device epoch (clearing `carry_words` on wrap) and zero
+// `ntickets` ticket counters (stride satdev::BATCH_TICKET_STRIDE), k_sat.hip.
+sat_status launch_bump_epoch(void* ws, uint64_t carry_words, hipStream_t stream, uint32_t* tickets = nullptr,
+                             uint32_t ntickets = 0);
 
 // Device address of the current device's sticky give-up flag in pinned host
 // memory (the host reads it without synchronising); null if it could not be

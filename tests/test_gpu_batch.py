@@ -141,3 +141,26 @@ def test_batch_graph_replay(O, dev, sa):
         g.replay()
         torch.cuda.synchronize()
         check_batch(O, imgs)
+
+
+@pytest.mark.parametrize("persistent", [1, 0])
+def test_batch_both_launch_forms(O, dev, sa, persistent):
+    """The persistent form (workgroups drawing queue items from per-XCD ticket
+    counters) and the one-workgroup-per-item form give the oracle's results,
+    on a batch with many more items than resident workgroups, run twice."""
+    say(f"[case] batch launch form persistent={persistent}")
+    L = sa.lib()
+    before = L.sat_batch_persistent_get()
+    assert L.sat_batch_persistent_set(persistent) == 0
+    try:
+        imgs = make_batch(O, dev, 96, 64, 1500, 23)
+        plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+        for rep in range(2):
+            for _, _, _, yd in imgs:
+                yd.zero_()
+            plan.run()
+            torch.cuda.synchronize()
+            check_batch(O, imgs)
+        plan.check_faults()
+    finally:
+        L.sat_batch_persistent_set(before)
