@@ -431,8 +431,13 @@ struct TileOp {
           q.w = cvt_out<TDst>(y[3]);
           TDst* o0 = out + (py0 + (REV ? TH - 1 - w * RPT : w * RPT)) * g.ldo + pcol;
           // streaming output: nontemporal stores keep it from displacing
-          // the carry words and the input in the caches
-          __builtin_nontemporal_store(q, (typename VecG<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo)));
+          // the carry words and the input in the caches. An 8-byte output
+          // row chunk is two 16-byte stores per lane, each covering half of
+          // every 64-byte line: plain stores, which L2 can merge, measured
+          // 3-5 % faster there (f64 4096^2, f32 -> f64 96 x 1024^2)
+          typename VecG<TDst>::type* oq = (typename VecG<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo));
+          if constexpr (sizeof(TDst) == 8) *oq = q;
+          else __builtin_nontemporal_store(q, oq);
         } else {
           store4<TDst>(out + prow * g.ldo, pcol, g.W, prow < (int64_t)g.H, y);
         }
