@@ -28,6 +28,47 @@ __device__ __forceinline__ void run_kind(const ImgArgs& g, uint32_t lt, uint32_t
   Op::run(g, lt, ep4, *reinterpret_cast<typename Op::Smem*>(smem));
 }
 
+// ImgArgs as the batch items hold it, with the pointers typed as global
+// memory. Pointers read from memory as plain (generic) pointers make the
+// compiler address the image through flat instructions, which also count
+// against the LDS wait counter (an LDS wait then also waits for outstanding
+// image accesses); read through this view, they are known to be global and
+// take global instructions, as kernel arguments do. (On the 512-image
+// workload the two forms measured the same time.)
+#define SAT_GLOBAL __attribute__((address_space(1)))
+struct ImgArgsG {
+  SAT_GLOBAL const void* in;
+  SAT_GLOBAL void* out;
+  int64_t ldi, ldo;
+  uint32_t H, W, nTx, nTy;
+  SAT_GLOBAL uint64_t* vcarry;
+  SAT_GLOBAL uint64_t* hcarry;
+  SAT_GLOBAL uint32_t* fault_ws;
+  SAT_GLOBAL uint32_t* fault_host;
+};
+static_assert(sizeof(ImgArgsG) == sizeof(ImgArgs) && offsetof(ImgArgsG, vcarry) == offsetof(ImgArgs, vcarry) &&
+                  offsetof(ImgArgsG, fault_ws) == offsetof(ImgArgs, fault) &&
+                  offsetof(ImgArgsG, fault_host) == offsetof(ImgArgs, fault) + sizeof(void*),
+              "ImgArgsG mirrors ImgArgs");
+
+__device__ __forceinline__ ImgArgs load_args(const ImgArgs* p) {
+  const ImgArgsG a = *reinterpret_cast<const ImgArgsG*>(p);
+  ImgArgs g;
+  g.in = (const void*)a.in;
+  g.out = (void*)a.out;
+  g.ldi = a.ldi;
+  g.ldo = a.ldo;
+  g.H = a.H;
+  g.W = a.W;
+  g.nTx = a.nTx;
+  g.nTy = a.nTy;
+  g.vcarry = (uint64_t*)a.vcarry;
+  g.hcarry = (uint64_t*)a.hcarry;
+  g.fault.ws = (uint32_t*)a.fault_ws;
+  g.fault.host = (uint32_t*)a.fault_host;
+  return g;
+}
+
 // Runs queue item q: tile queue[2q+1] of image queue[2q] (an empty item
 // returns at once). `sm` is the reference operator's shared memory, which
 // every operator of the launch's output width fits in.
@@ -38,7 +79,7 @@ __device__ __forceinline__ void run_item(const sathost::BatchItem* __restrict__ 
   if (img == 0xffffffffu) return;
   const uint32_t lt = __builtin_amdgcn_readfirstlane(queue[2 * q + 1]);
   const sathost::BatchItem& it = items[img];
-  const ImgArgs g = it.g;
+  const ImgArgs g = load_args(&it.g);
   const uint32_t kind = __builtin_amdgcn_readfirstlane(it.kind);
   if constexpr (OUTB == 4) {
     switch (kind) {

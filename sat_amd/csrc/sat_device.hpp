@@ -394,27 +394,33 @@ __device__ __forceinline__ void load4(const TSrc* __restrict__ row, int64_t col,
   }
 }
 
-// Edge tiles of images at least 4 wide: every lane issues one vector load
-// per row with no branch around it (so all rows' loads are in flight
-// together). `row` is the image row, already clamped into the image, and a
-// chunk that would cross the right edge reads the last whole chunk of the
-// row, [W-4, W), and shifts its elements into place; elements outside the
-// image (col + c >= W, or row_ok false) are zero.
+// Edge chunks of images at least 4 wide, in two steps so that a caller issues
+// every row's load before it uses any (branches between the loads made the
+// compiler wait for each load in turn: one memory round trip per row of an
+// edge tile). edge_base is the first element of the vector a lane loads: its
+// chunk [col, col+4), or the row's last whole chunk [W-4, W) when the chunk
+// crosses or lies past the right edge; rows are clamped into the image by the
+// caller. edge_pick shifts the loaded elements into place with selects and
+// zeroes those outside the image (col + c >= W, or row_ok false).
+__device__ __forceinline__ int64_t edge_base(int64_t col, int64_t W) { return col + 3 < W ? col : W - 4; }
+
 template <typename T>
-__device__ __forceinline__ T pick4(const T (&e)[4], int i) {
-  return i == 0 ? e[0] : i == 1 ? e[1] : i == 2 ? e[2] : e[3];
+__device__ __forceinline__ T sel4(const T (&e)[4], int i) {
+  const T a = (i & 1) ? e[1] : e[0];
+  const T b = (i & 1) ? e[3] : e[2];
+  return (i & 2) ? b : a;
 }
 
 template <typename TSrc, typename TDst, typename A>
-__device__ __forceinline__ void load4_edge(const TSrc* __restrict__ row, int64_t col, int64_t W, bool row_ok,
-                                           A (&x)[4]) {
-  const int64_t base = col + 3 < W ? col : W - 4;
-  const typename VecG<TSrc>::type q = *(const typename VecG<TSrc>::type*)(row + base);
+__device__ __forceinline__ void edge_pick(const typename VecG<TSrc>::type& q, int64_t col, int64_t base, int64_t W,
+                                          bool row_ok, A (&x)[4]) {
   const TSrc e[4] = {q.x, q.y, q.z, q.w};
   const int s = (int)(col - base);
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
-    x[c] = (row_ok && col + c < W) ? cvt_in<TSrc, TDst, A>(pick4(e, (c + s) & 3)) : zero<A>();
+  for (int c = 0; c < 4; ++c) {
+    const A val = cvt_in<TSrc, TDst, A>(sel4(e, (c + s) & 3));
+    x[c] = (row_ok && col + c < W) ? val : zero<A>();
+  }
 }
 
 template <typename TDst, typename A>

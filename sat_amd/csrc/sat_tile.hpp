@@ -159,13 +159,20 @@ struct TileOp {
         for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
       }
     } else if (g.W >= 4) {
-      // edge tile: clamped rows and chunks, one vector load per row
+      // edge tile: clamped rows and chunks, one vector load per row, every
+      // row's load issued before any is used
+      typename VecG<TSrc>::type q[RPT];
+      const int64_t base = edge_base(k.pcol, g.W);
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int64_t prow = prow_of(k, r);
-        const bool ok = prow < (int64_t)g.H;
+        const int64_t crow = prow < (int64_t)g.H ? prow : (int64_t)g.H - 1;
+        q[r] = *(const typename VecG<TSrc>::type*)(in + crow * g.ldi + base);
+      }
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
         A x[4];
-        load4_edge<TSrc, TDst>(in + (ok ? prow : (int64_t)g.H - 1) * g.ldi, k.pcol, g.W, ok, x);
+        edge_pick<TSrc, TDst>(q[r], k.pcol, base, g.W, prow_of(k, r) < (int64_t)g.H, x);
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
       }
@@ -700,13 +707,18 @@ struct SerialTileOp {
           x[i][3] = cvt_in<TSrc, TDst>(q.w);
         }
       } else if (g.W >= 4) {
-        // edge tile: clamped rows and chunks, one vector load per row
+        // edge tile: clamped rows and chunks, one vector load per row, every
+        // row's load issued before any is used
+        typename VecG<TSrc>::type q[NLOAD];
+        const int64_t base = edge_base(col, g.W);
 #pragma unroll
         for (int i = 0; i < NLOAD; ++i) {
           const int64_t row = y0 + r0 + i * RPL;
-          const bool ok = row < (int64_t)g.H;
-          load4_edge<TSrc, TDst>(in + (ok ? row : (int64_t)g.H - 1) * g.ldi, col, g.W, ok, x[i]);
+          q[i] = *(const typename VecG<TSrc>::type*)(in + (row < (int64_t)g.H ? row : (int64_t)g.H - 1) * g.ldi + base);
         }
+#pragma unroll
+        for (int i = 0; i < NLOAD; ++i)
+          edge_pick<TSrc, TDst>(q[i], col, base, g.W, y0 + r0 + i * RPL < (int64_t)g.H, x[i]);
       } else {
 #pragma unroll
         for (int i = 0; i < NLOAD; ++i) {
