@@ -46,6 +46,8 @@ template <> struct TileCfg<4> { static constexpr int NW = 4, RPT4 = 24, WPE = 3;
 // 5: 1-byte inputs into 4-byte outputs take PackedTileOp (4 waves x 64 packed
 //    rows, 256 x 256 tiles); every other pair takes configuration 1's tile
 template <> struct TileCfg<5> { static constexpr int NW = 8, RPT4 = 16, WPE = 4; };
+// 6: 4 waves x 32 rows (16 for 8-byte outputs) at 2 waves per SIMD
+template <> struct TileCfg<6> { static constexpr int NW = 4, RPT4 = 32, WPE = 2; };
 constexpr int PACKED_CFG = 5;
 template <typename S, typename D> constexpr bool packable = sizeof(S) == 1 && sizeof(arith_t<D>) == 4;
 template <typename S, typename D, int SC> using PackedOp = satdev::PackedTileOp<S, D, SC, 4, 64>;
@@ -198,5 +200,7 @@ sat_status launch_fam_sat_c4(const Launch& L, void* ws, size_t b, hipStream_t st
 sat_status launch_fam_sat_rev_c4(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_c5(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_rev_c5(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_c6(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev_c6(const Launch& L, void* ws, size_t b, hipStream_t st);
 
 }  // namespace sathost

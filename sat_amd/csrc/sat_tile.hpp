@@ -40,6 +40,10 @@ enum : int { MODE_ROW = 1, MODE_COL = 2, MODE_SAT = 3 };
 // past the inclusive one, so the windows are short.
 constexpr int LB_WINDOW = 2;
 constexpr int LB_WINDOW_V = 1;
+// Horizontal window per payload width: an 8-byte payload is two words, and 4
+// predecessors per round trip measured 8-12 % faster than 2 there (f64
+// 4096^2 and 8192^2; 8 was slower again), while 4-byte payloads lost with 4.
+template <typename A> constexpr int lb_window_h() { return sizeof(A) == 8 ? 4 : LB_WINDOW; }
 
 template <bool C, typename T, typename F> struct cond { using type = T; };
 template <typename T, typename F> struct cond<false, T, F> { using type = F; };
@@ -246,7 +250,7 @@ struct TileOp {
           if constexpr (KH == 1) {
             if (lane < TH) {
               const int64_t e = (int64_t)t * TH + lane;
-              rab[0] = look_back<A, LB_WINDOW>(g.hcarry, e - TH, TH, J, ep4, g.fault);
+              rab[0] = look_back<A, lb_window_h<A>()>(g.hcarry, e - TH, TH, J, ep4, g.fault);
               Carry<A>::put(g.hcarry, e, rab[0] + sm.rs[lane], ep4 | ST_INC);
             }
           } else {
@@ -257,7 +261,7 @@ struct TileOp {
               skip[k] = 64 * k + lane >= TH;
               e[k] = (int64_t)(t - 1) * TH + (skip[k] ? 0 : 64 * k + lane);
             }
-            look_back_k<A, LB_WINDOW, KH>(g.hcarry, e, TH, J, ep4, g.fault, rab, skip);
+            look_back_k<A, lb_window_h<A>(), KH>(g.hcarry, e, TH, J, ep4, g.fault, rab, skip);
 #pragma unroll
             for (int k = 0; k < KH; ++k)
               if (!skip[k])

@@ -28,7 +28,8 @@ CFG_NAMES = {0: "4 waves x 16 rows, 64x256 (32x256 for 8-byte outputs), 4 waves 
              2: "8 waves x 8 rows, 64x256 (32x256), up to 6 waves per SIMD",
              3: "8 waves x 8 rows, 64x256 (32x256), 8 waves per SIMD",
              4: "4 waves x 24 rows, 96x256 (48x256), 3 waves per SIMD",
-             5: "1-byte inputs into 4-byte outputs: 4 waves x 64 packed rows, 256x256; other pairs as 1"}
+             5: "1-byte inputs into 4-byte outputs: 4 waves x 64 packed rows, 256x256; other pairs as 1",
+             6: "4 waves x 32 rows, 128x256 (64x256), 2 waves per SIMD"}
 
 # the batch kernel's configurations (0-4; the packed configuration 5 is
 # one-pass only)
