@@ -72,6 +72,17 @@ inline uint64_t op_carry_words(const Launch& L) {
   return ntiles * (uint64_t)(Op::VWORDS + Op::HWORDS);
 }
 
+// Tile columns of one XCD run, given the `per` tile columns of a tile row
+// that one XCD takes: at most 4 (the largest divisor of `per` up to 4). A row
+// chain crosses XCDs between runs; runs of 4 measured 4 % faster than 8 on
+// u8 -> u32 16384^2 (64 tile columns) and than 32 on f32 256 x 65536, and
+// equal on the 8- and 16-wide XCD shares of f32 8192^2 and 16384^2.
+inline uint32_t run_length(uint32_t per) {
+  for (uint32_t r = 4; r > 1; --r)
+    if (per % r == 0) return r;
+  return 1;
+}
+
 template <class Op, int WPE = 4>
 inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStream_t stream) {
   const uint64_t nTx = (L.width + Op::TW - 1) / Op::TW, nTy = (L.height + Op::TH - 1) / Op::TH;
@@ -98,7 +109,7 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   g.fault.host = host_fault_word(L.may_alloc);
   satdev::TileOrder order;
   order.nx = (uint32_t)device_xcd_count();
-  order.run = (order.nx > 1 && nTx % order.nx == 0) ? (uint32_t)(nTx / order.nx) : 0u;
+  order.run = (order.nx > 1 && nTx % order.nx == 0) ? run_length((uint32_t)(nTx / order.nx)) : 0u;
   hipLaunchKernelGGL((satdev::tiles_kernel<Op, WPE>), dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, g, (uint32_t)tpp,
                      L.in_plane, L.out_plane, L.epoch, (const satdev::WsHeader*)ws, order);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;

@@ -808,11 +808,12 @@ struct SerialTileOp {
 // ---------------------------------------------------------------------------
 // One tile per workgroup. The hardware hands workgroup b to XCD b % nx and each
 // XCD starts its workgroups in increasing b. With order.run > 0 (tile rows
-// split into nx runs of `run` tiles), XCD x takes tile columns
-// [x*run, (x+1)*run) of every tile row, top to bottom: a tile's left and upper
-// neighbours then sit on its own XCD, started just before it, and the row
-// chains cross XCDs only between runs (in XCD order, so the dies settle into
-// a staggered pipeline instead of waiting on each other in a cycle). The map
+// split into runs of `run` tiles, run q of a row on XCD q % nx), XCD x takes
+// its runs of every tile row, top to bottom: a tile's upper neighbour, and
+// its left one inside a run, then sit on its own XCD, started just before it,
+// and the row chains cross XCDs only between runs (from XCD x-1 to x, so the
+// dies settle into a staggered pipeline instead of waiting on each other in a
+// cycle). The map
 // keeps every XCD's sequence in row-major tile order, which guarantees
 // progress (see the top of this file); the bounded look-back turns a
 // violation of the hardware's dispatch order into a reported fault instead of
