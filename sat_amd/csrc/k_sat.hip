@@ -130,6 +130,7 @@ sat_status launch_fam_sat(const Launch& L, void* ws, size_t b, hipStream_t st) {
     case 4: return launch_fam_sat_c4(L, ws, b, st);
     case 5: return launch_fam_sat_c5(L, ws, b, st);
     case 6: return launch_fam_sat_c6(L, ws, b, st);
+    case 7: return launch_fam_sat_c7(L, ws, b, st);
     default: return launch_sat_cfg<0>(L, ws, b, st);
   }
 }
@@ -142,6 +143,7 @@ sat_status launch_fam_sat_rev(const Launch& L, void* ws, size_t b, hipStream_t s
     case 4: return launch_fam_sat_rev_c4(L, ws, b, st);
     case 5: return launch_fam_sat_rev_c5(L, ws, b, st);
     case 6: return launch_fam_sat_rev_c6(L, ws, b, st);
+    case 7: return launch_fam_sat_rev_c7(L, ws, b, st);
     default: return launch_sat_rev_cfg<0>(L, ws, b, st);
   }
 }
@@ -158,6 +160,7 @@ Geometry geometry(int family, size_t out_size, int cfg, size_t in_size) {
         case 4: return sat_geometry_cfg<4>(out_size, in_size);
         case 5: return sat_geometry_cfg<5>(out_size, in_size);
         case 6: return sat_geometry_cfg<6>(out_size, in_size);
+        case 7: return sat_geometry_cfg<7>(out_size, in_size);
         default: return sat_geometry_cfg<0>(out_size, in_size);
       }
     case FAM_ROWS:

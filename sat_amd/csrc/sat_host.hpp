@@ -32,7 +32,7 @@ struct Geometry {
 // [0, TILE_CFGS), and of the batch kernel, cfg in [0, BATCH_CFGS); other
 // families have one shape. The one-pass geometry of configuration 5 depends
 // on the input width.
-constexpr int TILE_CFGS = 7;
+constexpr int TILE_CFGS = 8;
 constexpr int BATCH_CFGS = 5;
 Geometry geometry(int family, size_t out_size, int cfg = 0, size_t in_size = 4);
 

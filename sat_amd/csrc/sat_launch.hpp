@@ -48,9 +48,14 @@ template <> struct TileCfg<4> { static constexpr int NW = 4, RPT4 = 24, WPE = 3;
 template <> struct TileCfg<5> { static constexpr int NW = 8, RPT4 = 16, WPE = 4; };
 // 6: 4 waves x 32 rows (16 for 8-byte outputs) at 2 waves per SIMD
 template <> struct TileCfg<6> { static constexpr int NW = 4, RPT4 = 32, WPE = 2; };
-constexpr int PACKED_CFG = 5;
+// 7: 1-byte inputs into 4-byte outputs take PackedTileOp with 4 waves x 32
+//    packed rows (128 x 256 tiles); every other pair takes configuration 0's
+//    tile
+template <> struct TileCfg<7> { static constexpr int NW = 4, RPT4 = 16, WPE = 4; };
+// packed rows per wave of configuration CFG for 1-byte inputs (0: unpacked)
+template <int CFG> constexpr int packed_rpt = CFG == 5 ? 64 : CFG == 7 ? 32 : 0;
 template <typename S, typename D> constexpr bool packable = sizeof(S) == 1 && sizeof(arith_t<D>) == 4;
-template <typename S, typename D, int SC> using PackedOp = satdev::PackedTileOp<S, D, SC, 4, 64>;
+template <typename S, typename D, int SC, int RPT> using PackedOp = satdev::PackedTileOp<S, D, SC, 4, RPT>;
 template <int CFG, typename TDst>
 constexpr int cfg_rpt = sizeof(arith_t<TDst>) == 4 ? TileCfg<CFG>::RPT4 : TileCfg<CFG>::RPT4 / 2;
 
@@ -161,7 +166,7 @@ inline sat_status dispatch_pairs(const Launch& L, void* ws, size_t b, hipStream_
 // configuration lives in its own translation unit (k_sat*.hip).
 template <int CFG> struct FamSatCfg {
   template <typename S, typename D, int SC> struct Fwd {
-    using Op = cond_t<CFG == PACKED_CFG && packable<S, D>, PackedOp<S, D, SC>,
+    using Op = cond_t<(packed_rpt<CFG> > 0) && packable<S, D>, PackedOp<S, D, SC, packed_rpt<CFG> ? packed_rpt<CFG> : 64>,
                       satdev::TileOp<S, D, satdev::MODE_SAT, false, false, SC, TileCfg<CFG>::NW, cfg_rpt<CFG, D>>>;
     static sat_status go(const Launch& L, void* ws, size_t b, hipStream_t st) {
       return launch_op<Op, TileCfg<CFG>::WPE>(L, ws, b, st);
@@ -213,5 +218,7 @@ sat_status launch_fam_sat_c5(const Launch& L, void* ws, size_t b, hipStream_t st
 sat_status launch_fam_sat_rev_c5(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_c6(const Launch& L, void* ws, size_t b, hipStream_t st);
 sat_status launch_fam_sat_rev_c6(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_c7(const Launch& L, void* ws, size_t b, hipStream_t st);
+sat_status launch_fam_sat_rev_c7(const Launch& L, void* ws, size_t b, hipStream_t st);
 
 }  // namespace sathost

@@ -29,9 +29,14 @@ CFG_NAMES = {0: "4 waves x 16 rows, 64x256 (32x256 for 8-byte outputs), 4 waves 
              3: "8 waves x 8 rows, 64x256 (32x256), 8 waves per SIMD",
              4: "4 waves x 24 rows, 96x256 (48x256), 3 waves per SIMD",
              5: "1-byte inputs into 4-byte outputs: 4 waves x 64 packed rows, 256x256; other pairs as 1",
-             6: "4 waves x 32 rows, 128x256 (64x256), 2 waves per SIMD"}
+             6: "4 waves x 32 rows, 128x256 (64x256), 2 waves per SIMD",
+             7: "1-byte inputs into 4-byte outputs: 4 waves x 32 packed rows, 128x256; other pairs as 0"}
 
-# the batch kernel's configurations (0-4; the packed configuration 5 is
+# the packed configurations (1-byte inputs) and the configuration whose tile
+# they run for wider inputs
+PACKED_ALIAS = {5: 1, 7: 0}
+
+# the batch kernel's configurations (0-4; the packed configurations are
 # one-pass only)
 BATCH_CFGS = 5
 
@@ -139,6 +144,11 @@ def _tune_shape(L, sa, H, W, P, in_bytes, out_bytes, reps, log):
     ys = [torch.empty(shape, dtype=out_dt, device=dev) for _ in range(nbuf)]
     times, ref = [], None
     for cfg in range(L.sat_tile_config_count()):
+        if in_bytes != 1 and cfg in PACKED_ALIAS:
+            # the packed configurations run their unpacked tile for wider
+            # inputs: the same kernel, so the same time (ties pick the lower)
+            times.append(times[PACKED_ALIAS[cfg]])
+            continue
         sa.check(L.sat_tune_set(in_bytes, out_bytes, cx, cy, cp, cfg), "sat_tune_set")
         assert L.sat_tune_pick(in_bytes, out_bytes, W, H, P) == cfg
         sa.integral(x, out_dtype=out_dt, out=ys[0])

@@ -157,6 +157,45 @@ def test_managed_path_graph_replay(O, dev, sa):
     assert sa.lib().sat_workspace_fault(None, s.cuda_stream) == 0
 
 
+def test_graph_replay_after_workspace_grows(O, dev, sa):
+    """A graph captured on a small image keeps working after a larger eager
+    call on the same stream grew the managed workspace (the captured launch
+    still points at the old buffer, which the library retires instead of
+    freeing)."""
+    say("[case] graph replay after the managed workspace grew")
+    H, W = 200, 300
+    s = torch.cuda.Stream()
+    x = torch.empty((H, W), dtype=torch.uint8, device=dev)
+    y = torch.empty((H, W), dtype=torch.uint32, device=dev)
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        sa.integral(x.fill_(1), out=y)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        sa.integral(x, out=y)
+    big_np = O.gen("u8", (4096, 6000), 310)  # a workspace many times the small one's
+    big = torch.from_numpy(big_np).to(dev)
+    with torch.cuda.stream(s):
+        zb = sa.integral(big)
+    torch.cuda.synchronize()
+    zb_np = zb.view(torch.int32).cpu().numpy().view(np.uint32)
+    assert np.array_equal(zb_np[-1], O.serial(big_np, np.uint32)[-1])
+    del zb, zb_np
+    for rep in range(4):
+        xn = O.gen("u8", (H, W), 320 + rep)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            x.copy_(torch.from_numpy(xn).to(dev))
+            g.replay()
+            z = sa.integral(x)  # eager call on the grown workspace between replays
+        torch.cuda.synchronize()
+        exp = O.serial(xn, np.uint32)
+        assert np.array_equal(y.view(torch.int32).cpu().numpy().view(np.uint32), exp), f"replay {rep}"
+        assert np.array_equal(z.view(torch.int32).cpu().numpy().view(np.uint32), exp)
+    assert sa.lib().sat_workspace_fault(None, s.cuda_stream) == 0
+
+
 def test_box_grad_caller_workspace_many_planes(O, dev, sa):
     """The binned fused gradient (caller-provided workspace, many planes,
     boxes touching the borders) equals reverse SAT of the scattered corner

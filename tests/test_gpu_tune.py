@@ -41,7 +41,7 @@ def force(L, in_bytes, out_bytes, w, h, p, cfg):
     assert L.sat_tune_pick(in_bytes, out_bytes, w, h, p) == cfg
 
 
-CFGS = [0, 1, 2, 3, 4, 5, 6]
+CFGS = [0, 1, 2, 3, 4, 5, 6, 7]
 BATCH_CFGS = [0, 1, 2, 3, 4]
 IN_BYTES = {"u8": 1, "i8": 1, "u32": 4, "i32": 4, "f32": 4, "f64": 8}
 
@@ -115,19 +115,20 @@ def test_tuner_quick(sa, tmp_path):
     assert tune.apply_table(p) == 5
 
 
-# The packed 1-byte tiles (configuration 5): partial chunks at the right edge
-# (widths not a multiple of 4, below 4), partial 256-row tiles, rows that
+# The packed 1-byte tiles (configurations 5 and 7): partial chunks at the right edge
+# (widths not a multiple of 4, below 4), partial 256- and 128-row tiles, rows that
 # start at any byte (pitch and offset), negative i8 values, wraparound.
 PACKED_CASES = [((1, 1), 1, 0), ((2, 3), 3, 1), ((5, 7), 9, 2), ((300, 257), 260, 3), ((257, 1030), 1033, 1),
                 ((1000, 77), 80, 0), ((513, 4099), 4100, 0)]
 
 
+@pytest.mark.parametrize("cfg", [5, 7])
 @pytest.mark.parametrize("tin,tout", [("u8", "u32"), ("i8", "i32"), ("u8", "f32"), ("i8", "f32")])
-def test_packed_edges_pitches_offsets(O, dev, sa, tin, tout):
-    say(f"[case] packed tiles {tin}->{tout}")
+def test_packed_edges_pitches_offsets(O, dev, sa, tin, tout, cfg):
+    say(f"[case] packed tiles cfg {cfg} {tin}->{tout}")
     L = sa.lib()
     for k, ((h, w), pitch, off) in enumerate(PACKED_CASES):
-        force(L, 1, 4, w, h, 1, 5)
+        force(L, 1, 4, w, h, 1, cfg)
         x = O.gen(KIND[tin], (h, w), 5100 + k)
         buf = np.zeros(off + h * pitch + 8, dtype=x.dtype)
         buf[off:off + h * pitch].reshape(h, pitch)[:, :w] = x
@@ -142,11 +143,12 @@ def test_packed_edges_pitches_offsets(O, dev, sa, tin, tout):
     sa.check_faults()
 
 
-def test_packed_all_255_wraps(dev, sa):
-    say("[case] packed tiles u8 all 255, 4500 x 4500 (wraps modulo 2^32)")
+@pytest.mark.parametrize("cfg", [5, 7])
+def test_packed_all_255_wraps(dev, sa, cfg):
+    say(f"[case] packed tiles cfg {cfg} u8 all 255, 4500 x 4500 (wraps modulo 2^32)")
     L = sa.lib()
     H = W = 4500
-    force(L, 1, 4, W, H, 1, 5)
+    force(L, 1, 4, W, H, 1, cfg)
     x = torch.full((H, W), 255, dtype=torch.uint8, device=dev)
     y = sa.integral(x, out_dtype=torch.uint32).view(torch.int32).to(torch.int64) & 0xFFFFFFFF
     i1 = torch.arange(1, H + 1, dtype=torch.int64, device=dev)[:, None]

@@ -93,7 +93,7 @@ def test_builtin_choice(L):
     from sat_amd.tune import reset_table
 
     reset_table(L)
-    assert L.sat_tile_config_count() == 7
+    assert L.sat_tile_config_count() == 8
     # narrow and tall: 8-wave double-height tiles; square and wide: 4-wave tiles
     assert L.sat_tune_pick(4, 4, 256, 65536, 1) == 1
     assert L.sat_tune_pick(4, 4, 8192, 8192, 1) == 0
@@ -130,7 +130,7 @@ def test_invalid_arguments(L):
     assert L.sat_tune_set(4, 4, 9, 0, 0, 0) != 0
     assert L.sat_tune_set(4, 4, 0, 12, 0, 0) != 0
     assert L.sat_tune_set(4, 4, 0, 0, 4, 0) != 0
-    assert L.sat_tune_set(4, 4, 0, 0, 0, 7) != 0
+    assert L.sat_tune_set(4, 4, 0, 0, 0, 8) != 0
     assert L.sat_tune_set(4, 4, -1, 0, 0, 0) != 0
     assert L.sat_tune_get(4, 16, 0, 0, 0) < 0
     assert L.sat_tune_pick(8, 4, 64, 64, 1) < 0
@@ -142,12 +142,12 @@ def test_workspace_fits_every_config(L):
     # workspace queries do not depend on the table (configuration 5's packed
     # tiles of 1-byte inputs included)
     b8 = L.sat_integral_workspace_bytes(3, 0, 4, 1000, 70000)
-    for cfg in range(7):
+    for cfg in range(8):
         L.sat_tune_set(1, 4, *py_class(1000, 70000, 1), cfg)
         assert L.sat_integral_workspace_bytes(3, 0, 4, 1000, 70000) == b8
     L.sat_tune_set(1, 4, *py_class(1000, 70000, 1), -1)
     before = L.sat_integral_workspace_bytes(3, 6, 6, 1000, 70000)
-    for cfg in range(7):
+    for cfg in range(8):
         L.sat_tune_set(4, 4, *py_class(1000, 70000, 1), cfg)
         assert L.sat_integral_workspace_bytes(3, 6, 6, 1000, 70000) == before
     L.sat_tune_set(4, 4, *py_class(1000, 70000, 1), -1)
