@@ -179,22 +179,35 @@ template <typename A> struct Carry {
     }
   }
 
+  // The raw word(s) of entry e. Loads and their use are separate steps so
+  // that a look-back round issues every load of its window before it waits
+  // for any (a use right after each load costs one memory round trip per
+  // word).
+  struct Raw {
+    uint64_t w[WORDS];
+  };
+  __device__ static __forceinline__ Raw load(const uint64_t* base, int64_t e) {
+    Raw r;
+#pragma unroll
+    for (int i = 0; i < WORDS; ++i) r.w[i] = ld_word(base + WORDS * e + i);
+    return r;
+  }
   // 0: not published in this launch yet, otherwise ST_AGG / ST_INC
-  __device__ static __forceinline__ uint32_t get(const uint64_t* base, int64_t e, uint32_t ep4, A& v) {
+  __device__ static __forceinline__ uint32_t decode(const Raw& r, uint32_t ep4, A& v) {
     if constexpr (WORDS == 1) {
-      const uint64_t w = ld_word(base + e);
-      const uint32_t tag = (uint32_t)(w >> 32);
+      const uint32_t tag = (uint32_t)(r.w[0] >> 32);
       if ((tag & ~3u) != ep4) return 0;
-      v = __builtin_bit_cast(A, (uint32_t)w);
+      v = __builtin_bit_cast(A, (uint32_t)r.w[0]);
       return tag & 3u;
     } else {
-      const uint64_t h = ld_word(base + 2 * e);
-      const uint64_t l = ld_word(base + 2 * e + 1);
-      const uint32_t th = (uint32_t)(h >> 32), tl = (uint32_t)(l >> 32);
+      const uint32_t th = (uint32_t)(r.w[0] >> 32), tl = (uint32_t)(r.w[1] >> 32);
       if (th != tl || (th & ~3u) != ep4) return 0;  // torn AGG/INC pair: retry
-      v = __builtin_bit_cast(A, ((uint64_t)(uint32_t)h << 32) | (uint32_t)l);
+      v = __builtin_bit_cast(A, ((uint64_t)(uint32_t)r.w[0] << 32) | (uint32_t)r.w[1]);
       return th & 3u;
     }
+  }
+  __device__ static __forceinline__ uint32_t get(const uint64_t* base, int64_t e, uint32_t ep4, A& v) {
+    return decode(load(base, e), ep4, v);
   }
 };
 
@@ -257,11 +270,19 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
   while (n > 0) {
     A v[D];
     uint32_t st[D];
+    typename Carry<A>::Raw raw[D];
+    // every load of the window first (a window slot past the last
+    // predecessor re-reads that one and is ignored), then their use
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      st[d] = 0;
+      const uint32_t dd = (uint32_t)d < n ? (uint32_t)d : n - 1;
+      raw[d] = Carry<A>::load(base, e - (int64_t)dd * step);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
       v[d] = zero<A>();
-      if ((uint32_t)d < n) st[d] = Carry<A>::get(base, e - (int64_t)d * step, ep4, v[d]);
+      st[d] = Carry<A>::decode(raw[d], ep4, v[d]);
+      if ((uint32_t)d >= n) st[d] = 0;
     }
     uint32_t used = 0;
     bool done = false;
@@ -285,10 +306,13 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
 }
 
 // K independent look-back chains per lane over the same predecessors (for
-// example two columns of one tile): every round issues the windows of all K
-// chains together, so the chains share their memory round trips. Chains with
-// skip[k] set read nothing and leave acc[k] = 0.
-template <typename A, int D, int K>
+// example two columns of one tile). With BATCH every round issues the windows
+// of all K chains before it uses any, so the chains share one memory round
+// trip per round (finished chains and window slots past the last predecessor
+// re-read a word they are known to reach and ignore it); without it, each
+// chain's window is read and used in turn, one round trip per chain and
+// window slot. Chains with skip[k] set take no part and leave acc[k] = 0.
+template <typename A, int D, int K, bool BATCH = true>
 __device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t (&e0)[K], int64_t step, uint32_t n0,
                                             uint32_t ep4, const Fault& fault, A (&acc)[K],
                                             const bool* skip = nullptr) {
@@ -310,14 +334,36 @@ __device__ __forceinline__ void look_back_k(const uint64_t* base, const int64_t 
     if (all) break;
     A v[K][D];
     uint32_t st[K][D];
+    typename Carry<A>::Raw raw[K][D];
+    // every load of the round first, then their use: a finished chain
+    // re-reads its first predecessor and a window slot past the last
+    // predecessor re-reads that one; both are ignored
+    if constexpr (BATCH) {
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+      for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        st[k][d] = 0;
-        v[k][d] = zero<A>();
-        if (!fin[k] && (uint32_t)d < n[k]) st[k][d] = Carry<A>::get(base, e[k] - (int64_t)d * step, ep4, v[k][d]);
-      }
+        for (int d = 0; d < D; ++d) {
+          const uint32_t dd = (uint32_t)d < n[k] ? (uint32_t)d : n[k] - 1;
+          raw[k][d] = Carry<A>::load(base, fin[k] ? e0[k] : e[k] - (int64_t)dd * step);
+        }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          v[k][d] = zero<A>();
+          st[k][d] = Carry<A>::decode(raw[k][d], ep4, v[k][d]);
+          if (fin[k] || (uint32_t)d >= n[k]) st[k][d] = 0;
+        }
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          st[k][d] = 0;
+          v[k][d] = zero<A>();
+          if (!fin[k] && (uint32_t)d < n[k]) st[k][d] = Carry<A>::get(base, e[k] - (int64_t)d * step, ep4, v[k][d]);
+        }
+    }
     bool progress = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
