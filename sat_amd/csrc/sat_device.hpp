@@ -257,8 +257,9 @@ __device__ __forceinline__ bool wait_lead(const uint64_t* base, int64_t e, uint3
 
 // Decoupled look-back over `n` predecessors, nearest first. `e` is the carry
 // entry of the nearest predecessor, `step` the entry distance to the next one.
-// Each round issues the loads of a window of D predecessors at once (one
-// memory round trip), then folds the longest published prefix of the window:
+// Each round reads a window of D predecessors (for one-word payloads all its
+// loads at once, one memory round trip), then folds the longest published
+// prefix of the window:
 // aggregates are summed until an inclusive prefix ends the walk. A window
 // whose nearest word is not yet published is retried once wait_lead has
 // seen it published.
@@ -270,19 +271,31 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
   while (n > 0) {
     A v[D];
     uint32_t st[D];
-    typename Carry<A>::Raw raw[D];
-    // every load of the window first (a window slot past the last
-    // predecessor re-reads that one and is ignored), then their use
+    if constexpr (Carry<A>::WORDS == 1) {
+      // every load of the window first (a window slot past the last
+      // predecessor re-reads that one and is ignored), then their use
+      typename Carry<A>::Raw raw[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const uint32_t dd = (uint32_t)d < n ? (uint32_t)d : n - 1;
-      raw[d] = Carry<A>::load(base, e - (int64_t)dd * step);
-    }
+      for (int d = 0; d < D; ++d) {
+        const uint32_t dd = (uint32_t)d < n ? (uint32_t)d : n - 1;
+        raw[d] = Carry<A>::load(base, e - (int64_t)dd * step);
+      }
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      v[d] = zero<A>();
-      st[d] = Carry<A>::decode(raw[d], ep4, v[d]);
-      if ((uint32_t)d >= n) st[d] = 0;
+      for (int d = 0; d < D; ++d) {
+        v[d] = zero<A>();
+        st[d] = Carry<A>::decode(raw[d], ep4, v[d]);
+        if ((uint32_t)d >= n) st[d] = 0;
+      }
+    } else {
+      // two-word payloads: one window slot after the other (all at once
+      // measured 9 % slower on a 512-image f64 batch, and within 2 % on
+      // single f64 images)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        st[d] = 0;
+        v[d] = zero<A>();
+        if ((uint32_t)d < n) st[d] = Carry<A>::get(base, e - (int64_t)d * step, ep4, v[d]);
+      }
     }
     uint32_t used = 0;
     bool done = false;
