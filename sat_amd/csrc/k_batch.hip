@@ -126,9 +126,14 @@ __global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW))
 // own list, which workgroups that are running have drawn: the walk progresses
 // whatever the number of resident workgroups. The counters are zeroed by
 // bump_epoch_kernel before every launch.
+// The item loop around the inlined tile operators needs more registers than
+// the one-item kernel: at its configuration's waves per SIMD it spilled 60-135
+// registers and took 1.75 ms on the 512-image workload; with a register budget
+// of 2 waves per SIMD it spills none and takes 1.14 ms (the one-item form:
+// 0.91 ms).
 template <int OUTB, int CFG>
 __global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW))
-    __attribute__((amdgpu_waves_per_eu(BatchCfg<OUTB, CFG>::WPE)))
+    __attribute__((amdgpu_waves_per_eu(BatchCfg<OUTB, CFG>::WPE < 2 ? BatchCfg<OUTB, CFG>::WPE : 2)))
     batch_persistent_kernel(const WsHeader* hdr, const sathost::BatchItem* __restrict__ items,
                             const uint32_t* __restrict__ queue, uint32_t* tickets, uint32_t L, uint32_t nx) {
   using Cfg = BatchCfg<OUTB, CFG>;
