@@ -64,6 +64,7 @@ using GradOp = TileOp<T, T, MODE_SAT, true, false, 0, GradCfg<T>::NW, GradCfg<T>
 // physical tile i of plane p (entries of plane p start at p*4*nbox).
 constexpr int BIN_THREADS = 1024;
 constexpr uint32_t BIN_MAX_TILES = 8191;  // LDS counters per plane
+static_assert(BIN_THREADS % 64 == 0 && BIN_THREADS / 64 <= 64, "one wave scans the wave totals");
 
 template <typename T>
 __global__ void __launch_bounds__(BIN_THREADS)
@@ -73,7 +74,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
   using Op = GradOp<T>;
   constexpr int TH = Op::TH, TW = Op::TW;
   __shared__ uint32_t cnt[BIN_MAX_TILES + 1];
-  __shared__ uint32_t part[BIN_THREADS];
+  __shared__ uint32_t wsum[BIN_THREADS / 64];
   const uint32_t p = blockIdx.x, tid = threadIdx.x;
   const int4* pb = boxes + (uint64_t)p * nbox;
   const T* pg = gout + (uint64_t)p * nbox;
@@ -92,21 +93,22 @@ __global__ void __launch_bounds__(BIN_THREADS)
   }
   __syncthreads();
   // exclusive scan of cnt[0..tpp): each thread folds a run, then the runs
+  // are scanned across the block (each wave its 64 runs, one wave the wave
+  // totals)
   const uint32_t per = (tpp + BIN_THREADS - 1) / BIN_THREADS, i0 = tid * per;
   uint32_t run = 0;
   for (uint32_t i = i0; i < i0 + per && i < tpp; ++i) run += cnt[i];
-  part[tid] = run;
+  const uint32_t lane = tid & 63, wv = tid >> 6;
+  const uint32_t inc = wave_scan_ks(run);
+  if (lane == 63) wsum[wv] = inc;
   __syncthreads();
-  if (tid == 0) {
-    uint32_t acc = 0;
-    for (int k = 0; k < BIN_THREADS; ++k) {
-      const uint32_t v = part[k];
-      part[k] = acc;
-      acc += v;
-    }
+  if (wv == 0) {
+    const uint32_t t = lane < BIN_THREADS / 64 ? wsum[lane] : 0u;
+    const uint32_t ti = wave_scan_ks(t);
+    if (lane < BIN_THREADS / 64) wsum[lane] = ti - t;
   }
   __syncthreads();
-  uint32_t acc = part[tid];
+  uint32_t acc = wsum[wv] + inc - run;
   for (uint32_t i = i0; i < i0 + per && i < tpp; ++i) {
     const uint32_t v = cnt[i];
     cnt[i] = acc;
