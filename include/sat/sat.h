@@ -158,8 +158,10 @@ int sat_scan_cols(int scan, int tin, int tout, const void* in, int64_t in_pitch,
  * (IntegralImageSerial, integralImage.h:105-124). `in` and `out` are
  * height x width, pitches in elements of their own types. `workspace` may be
  * NULL (library-managed) or caller device memory of at least
- * sat_integral_workspace_bytes() bytes, zero-filled once before its first
- * use and not shared by calls that can overlap.
+ * sat_integral_workspace_bytes() bytes, 16-byte aligned (SAT_ERR_INVALID
+ * otherwise; hipMalloc memory is), zero-filled once before its first use and
+ * not shared by calls that can overlap. The same holds for the workspaces of
+ * the reverse SAT, the batch and sat_box_grad.
  * ------------------------------------------------------------------------- */
 size_t sat_integral_workspace_bytes(int alg, int tin, int tout, uint32_t width, uint32_t height);
 int sat_integral(int alg, int scan, int tin, int tout, const void* in, int64_t in_pitch, void* out,

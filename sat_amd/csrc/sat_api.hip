@@ -284,6 +284,7 @@ static bool stream_capturing(hipStream_t st) {
 static sat_status carry_open(hipStream_t st, void* user_ws, size_t user_bytes, size_t need, CarryCtx* c) {
   if (user_ws) {
     if (user_bytes < need) return SAT_ERR_WORKSPACE;
+    if ((uintptr_t)user_ws % 16) return SAT_ERR_INVALID;  // carry pairs are 16-byte accesses
     if (need > (size_t)satdev::WS_HEADER_BYTES) {
       const sat_status s = launch_clear((char*)user_ws + satdev::WS_HEADER_BYTES, need - satdev::WS_HEADER_BYTES, st);
       if (s != SAT_OK) return s;
@@ -940,6 +941,7 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
   sat_status s = batch_layout(images, count, -1, &info, &items, &queue);
   if (s != SAT_OK) return s;
   if (workspace_bytes < info.total) return SAT_ERR_WORKSPACE;
+  if ((uintptr_t)workspace % 16) return SAT_ERR_INVALID;
   const Geometry g = batch_geometry(info.out_size, info.cfg);
   uint64_t* carries = (uint64_t*)((char*)workspace + satdev::WS_HEADER_BYTES);
   for (auto& it : items) {

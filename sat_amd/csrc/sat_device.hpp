@@ -164,6 +164,20 @@ __device__ __forceinline__ uint64_t ld_word(const uint64_t* p) {
 __device__ __forceinline__ void st_word(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// The two words of an 8-byte payload move as one 16-byte access (a volatile
+// global dwordx4 at system scope, which like the agent-scope words reaches
+// memory on every store and load). Whether or not the hardware splits it, a
+// reader that sees halves of two different writes finds two different tags,
+// since each entry is written once per (epoch, status), and reads again. One
+// request per entry instead of two measured f64 8192^2 260 -> 245 us and the
+// f32 -> f64 table of 96 x 1024^2 296 -> 285 us.
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u64x2 ld_pair(const uint64_t* p) {
+  return *(const volatile __attribute__((address_space(1))) u64x2*)p;
+}
+__device__ __forceinline__ void st_pair(uint64_t* p, u64x2 v) {
+  *(volatile __attribute__((address_space(1))) u64x2*)p = v;
+}
 
 template <typename A> struct Carry {
   static constexpr int WORDS = sizeof(A) / 4;
@@ -174,8 +188,10 @@ template <typename A> struct Carry {
       st_word(base + e, t | __builtin_bit_cast(uint32_t, v));
     } else {
       const uint64_t b = __builtin_bit_cast(uint64_t, v);
-      st_word(base + 2 * e, t | (uint32_t)(b >> 32));
-      st_word(base + 2 * e + 1, t | (uint32_t)b);
+      u64x2 w;
+      w.x = t | (uint32_t)(b >> 32);
+      w.y = t | (uint32_t)b;
+      st_pair(base + 2 * e, w);
     }
   }
 
@@ -188,8 +204,13 @@ template <typename A> struct Carry {
   };
   __device__ static __forceinline__ Raw load(const uint64_t* base, int64_t e) {
     Raw r;
-#pragma unroll
-    for (int i = 0; i < WORDS; ++i) r.w[i] = ld_word(base + WORDS * e + i);
+    if constexpr (WORDS == 1) {
+      r.w[0] = ld_word(base + e);
+    } else {
+      const u64x2 w = ld_pair(base + 2 * e);
+      r.w[0] = w.x;
+      r.w[1] = w.y;
+    }
     return r;
   }
   // 0: not published in this launch yet, otherwise ST_AGG / ST_INC
@@ -289,7 +310,7 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
     } else {
       // two-word payloads: one window slot after the other (all at once
       // measured 9 % slower on a 512-image f64 batch, and within 2 % on
-      // single f64 images)
+      // single f64 images; the same with one 16-byte load per slot)
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         st[d] = 0;

@@ -230,6 +230,37 @@ def test_concurrent_streams(O, dev, sa):
         assert sa.lib().sat_workspace_fault(None, s.cuda_stream) == 0
 
 
+def test_caller_workspace_alignment(O, dev, sa):
+    """A caller workspace at a 16-byte offset into a garbage-filled buffer
+    (header zeroed) gives the oracle's results for 4- and 8-byte carries
+    (8-byte payloads are 16-byte carry pairs), twice in a row; one at an
+    8-byte offset is refused with SAT_ERR_INVALID before any launch."""
+    import ctypes as C
+
+    say("[case] caller workspace alignment")
+    L = sa.lib()
+    st = torch.cuda.current_stream().cuda_stream
+    for tin, tout, (h, w) in (("f64", "f64", (700, 1500)), ("u32", "u32", (700, 1500)), ("f32", "f64", (300, 900))):
+        nb = L.sat_integral_workspace_bytes(3, sa.sat_dtype(TD[tin]), sa.sat_dtype(TD[tout]), w, h)
+        raw = torch.full((nb + 64,), 0x5A, dtype=torch.uint8, device=dev)
+        base = raw.data_ptr()
+        x = O.gen(KIND[tin], (h, w), 8100)
+        xd = to_dev(x, dev)
+        y = torch.empty((h, w), dtype=TD[tout], device=dev)
+        args = lambda ws: (3, 0, sa.sat_dtype(TD[tin]), sa.sat_dtype(TD[tout]), C.c_void_p(xd.data_ptr()), w,
+                           C.c_void_p(y.data_ptr()), w, w, h, C.c_void_p(ws), nb, C.c_void_p(st))
+        assert base % 16 == 0
+        assert L.sat_integral(*args(base + 8)) == 2  # SAT_ERR_INVALID
+        ws = base + 16
+        raw[16:16 + 256].zero_()
+        for _ in range(2):
+            y.view(torch.uint8).fill_(0)
+            assert L.sat_integral(*args(ws)) == 0
+            torch.cuda.synchronize()
+            assert_sat_close(O, x, to_np(y), tout)
+        assert L.sat_workspace_fault(C.c_void_p(ws), C.c_void_p(st)) == 0
+
+
 def test_random_shapes_and_pitches(O, dev, sa):
     """A seeded sweep of random shapes, input row pitches and offsets, dtype
     pairs and algorithms (through the C ABI with explicit pitches), each
