@@ -164,3 +164,67 @@ def test_batch_both_launch_forms(O, dev, sa, persistent):
         plan.check_faults()
     finally:
         L.sat_batch_persistent_set(before)
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_device_epoch_wrap(O, dev, sa, persistent):
+    """The device epoch range [2^29, 2^30 - 1] wraps. Carry words of a first
+    run (epoch 2^29) are kept with their tags but garbage payloads; a run at
+    epoch 2^30 - 1 must ignore them (another epoch), and the run after it,
+    which wraps back to 2^29, must clear them before its tiles can read them.
+    Both runs give the oracle's results and no look-back gives up."""
+    say(f"[case] batch device epoch wrap persistent={persistent}")
+    L = sa.lib()
+    before = L.sat_batch_persistent_get()
+    assert L.sat_batch_persistent_set(persistent) == 0
+    try:
+        base, emax = 1 << 29, (1 << 30) - 1
+        imgs = make_batch(O, dev, 24, 1, 600, 31)
+        plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+        plan.run()
+        torch.cuda.synchronize()
+        check_batch(O, imgs)
+        ws = plan.workspace
+        assert ws.numel() % 8 == 0
+        assert int(ws.view(torch.int32)[0]) == base  # header epoch after the first device-epoch run
+
+        # The carry words lie between the 256-byte header and the item table,
+        # which starts with image 0's input and output pointers. Only words
+        # there are rewritten: the queue after the table holds words (empty
+        # entries, all ones) whose upper half reads like an epoch tag.
+        x0, y0 = imgs[0][2].data_ptr(), imgs[0][3].data_ptr()
+
+        def plant(from_epoch, header):
+            # every carry word tagged `from_epoch` -> tag of epoch 2^29 (status kept), garbage payload
+            w = ws.cpu().numpy().view(np.uint64).copy()
+            at = np.nonzero((w[:-1] == np.uint64(x0)) & (w[1:] == np.uint64(y0)))[0]
+            assert len(at) == 1 and at[0] > 32
+            idx = np.arange(w.size)
+            tag = w >> np.uint64(32)
+            st = tag & np.uint64(3)
+            hit = ((idx >= 32) & (idx < int(at[0])) & ((tag >> np.uint64(2)) == np.uint64(from_epoch))
+                   & ((st == np.uint64(1)) | (st == np.uint64(2))))
+            assert int(hit.sum()) > 100
+            planted = (((tag & np.uint64(3)) | np.uint64(base << 2)) << np.uint64(32)) | np.uint64(0x0BADF00D)
+            w = np.where(hit, planted, w)
+            ws.copy_(torch.from_numpy(w.view(np.uint8)).to(dev))
+            ws.view(torch.int32)[0] = header
+            torch.cuda.synchronize()
+
+        plant(base, emax - 1)  # next run: epoch 2^30 - 1, no clear
+        for _, _, _, yd in imgs:
+            yd.zero_()
+        plan.run()
+        torch.cuda.synchronize()
+        check_batch(O, imgs)
+        assert int(ws.view(torch.int32)[0]) == emax
+        plant(emax, emax)  # next run wraps to 2^29: must clear the planted words first
+        for _, _, _, yd in imgs:
+            yd.zero_()
+        plan.run()
+        torch.cuda.synchronize()
+        check_batch(O, imgs)
+        assert int(ws.view(torch.int32)[0]) == base
+        plan.check_faults()
+    finally:
+        L.sat_batch_persistent_set(before)
