@@ -116,6 +116,49 @@ def _time(fn, reps: int, nbuf: int = 1) -> float:
     return ts[len(ts) // 2]
 
 
+_GRAPH_STREAM = None
+
+
+def _time_graph(fn, reps: int, nbuf: int = 1) -> float:
+    """Median microseconds per call of `reps` replays of a hipGraph holding
+    max(8, nbuf) calls fn(k), k cycling over nbuf buffer sets. A replay
+    launches every call without host work in between, so images whose
+    kernels run shorter than a call's host path (a few tens of
+    microseconds) are timed on the GPU rather than on the host. Captured
+    calls advance the workspace's device epoch with a one-workgroup kernel
+    before each launch, the same for every tile configuration."""
+    import torch
+
+    global _GRAPH_STREAM
+    n = max(8, nbuf)
+    if _GRAPH_STREAM is None:
+        _GRAPH_STREAM = torch.cuda.Stream()  # one capture stream: one managed workspace
+    st = _GRAPH_STREAM
+    st.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(st):
+        for k in range(nbuf):
+            fn(k)  # the capture stream's managed workspace exists before capture
+    torch.cuda.current_stream().wait_stream(st)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=st):
+        for k in range(n):
+            fn(k % nbuf)
+    g.replay()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        g.replay()
+        e.record()
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e) * 1000 / n)
+    del g
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
 # timed calls rotate over input/output sets spanning at least this many bytes,
 # so that no call finds its input in the 256 MiB MALL
 ROTATE_BYTES = 1 << 30
@@ -164,7 +207,7 @@ def _tune_shape(L, sa, H, W, P, in_bytes, out_bytes, reps, log):
             err = float(((got.double() - ref.double()).abs() / scale.clamp_min(1e-300)).max())
             if err > 64 * 2.0 ** -24:
                 raise AssertionError(f"tile configuration {cfg} differs by {err:.2e} (relative) at {P}x{H}x{W}")
-        times.append(_time(lambda k: sa.integral(xs[k], out_dtype=out_dt, out=ys[k]), reps, nbuf))
+        times.append(_time_graph(lambda k: sa.integral(xs[k], out_dtype=out_dt, out=ys[k]), reps, nbuf))
     sa.check_faults()
     best = min(range(len(times)), key=lambda c: times[c])
     log(f"[tune] {P}x{H}x{W} {in_bytes}->{out_bytes}B class ({cx},{cy},{cp}): "
