@@ -259,6 +259,29 @@ def test_caller_workspace_alignment(O, dev, sa):
             torch.cuda.synchronize()
             assert_sat_close(O, x, to_np(y), tout)
         assert L.sat_workspace_fault(C.c_void_p(ws), C.c_void_p(st)) == 0
+    # the batch and the box gradient refuse an 8-byte-aligned workspace the same way
+    x = torch.zeros((40, 300), dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    img = sa.sat_image(x.data_ptr(), y.data_ptr(), 300, 300, 300, 40, sa.sat_dtype(x.dtype), sa.sat_dtype(y.dtype))
+    pimg = C.cast(C.pointer(img), C.c_void_p)
+    nb = L.sat_batch_workspace_bytes(pimg, 1)
+    raw = torch.zeros(nb + 64, dtype=torch.uint8, device=dev)
+    assert L.sat_batch_prepare(pimg, 1, C.c_void_p(raw.data_ptr() + 8), nb, C.c_void_p(st)) == 2
+    assert L.sat_batch_prepare(pimg, 1, C.c_void_p(raw.data_ptr() + 16), nb, C.c_void_p(st)) == 0
+    boxes = torch.tensor([[0, 0, 9, 9]], dtype=torch.int32, device=dev)
+    g = torch.ones(1, dtype=torch.float32, device=dev)
+    gx = torch.empty((40, 300), dtype=torch.float32, device=dev)
+    nb = L.sat_box_grad_workspace_bytes(sa.sat_dtype(torch.float32), 300, 40, 1, 1)
+    raw = torch.zeros(nb + 64, dtype=torch.uint8, device=dev)
+    args = lambda off: (sa.sat_dtype(torch.float32), C.c_void_p(boxes.data_ptr()), 1, C.c_void_p(g.data_ptr()),
+                        C.c_void_p(gx.data_ptr()), 300, 300 * 40, 300, 40, 1, C.c_void_p(raw.data_ptr() + off), nb,
+                        C.c_void_p(st))
+    assert L.sat_box_grad(*args(8)) == 2
+    assert L.sat_box_grad(*args(16)) == 0
+    torch.cuda.synchronize()
+    exp = torch.zeros_like(gx)
+    exp[:10, :10] = 1
+    assert torch.equal(gx, exp)
 
 
 def test_random_shapes_and_pitches(O, dev, sa):
