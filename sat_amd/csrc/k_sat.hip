@@ -176,7 +176,7 @@ Geometry geometry(int family, size_t out_size, int cfg, size_t in_size) {
       return b8 ? geom_of<SerialTileOp<double, double, kNW_S, kSerialCW<double>>>()
                 : geom_of<SerialTileOp<float, float, kNW_S, kSerialCW<float>>>();
     default: {
-      Geometry g{0, 0, 0, 0, 0};
+      Geometry g{0, 0, 0, 0, 0, 0};
       return g;
     }
   }

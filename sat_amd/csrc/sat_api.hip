@@ -43,7 +43,14 @@ static uint64_t tiles_of(const Geometry& g, uint32_t width, uint32_t height, uin
 size_t carry_bytes(const Launch& L) {
   const Geometry g = geometry(L.family, dtype_size(L.tout), L.cfg, dtype_size(L.tin));
   const uint64_t n = tiles_of(g, L.width, L.height, L.planes);
-  return (size_t)(satdev::WS_HEADER_BYTES + n * (uint64_t)(g.vwords + g.hwords) * 8);
+  uint64_t words = n * (uint64_t)(g.vwords + g.hwords);
+  if (g.gwords && n) {
+    // the second vertical carry level of tall images (as launch_op lays it out)
+    const uint64_t nTx = (L.width + g.tw - 1) / g.tw, nTy = (L.height + g.th - 1) / g.th;
+    const uint32_t G = vgroup_len(nTx, nTy);
+    if (G) words += (uint64_t)L.planes * ((nTy + G - 1) / G) * nTx * (uint64_t)g.gwords;
+  }
+  return (size_t)(satdev::WS_HEADER_BYTES + words * 8);
 }
 
 static bool tiled_by_cfg(int family) { return family == FAM_SAT || family == FAM_SAT_REV; }

@@ -283,12 +283,14 @@ __device__ __forceinline__ bool wait_lead(const uint64_t* base, int64_t e, uint3
 // prefix of the window:
 // aggregates are summed until an inclusive prefix ends the walk. A window
 // whose nearest word is not yet published is retried once wait_lead has
-// seen it published.
+// seen it published. With `got_inc`, reports whether an inclusive word ended
+// the walk (false: all n predecessors were aggregates, or a give-up).
 template <typename A, int D>
 __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t step, uint32_t n, uint32_t ep4,
-                                       const Fault& fault) {
+                                       const Fault& fault, bool* got_inc = nullptr) {
   A acc = zero<A>();
   uint32_t spins = 0;
+  if (got_inc) *got_inc = false;
   while (n > 0) {
     A v[D];
     uint32_t st[D];
@@ -328,7 +330,10 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
         done = st[d] == ST_INC;
       }
     }
-    if (done) break;
+    if (done) {
+      if (got_inc) *got_inc = true;
+      break;
+    }
     if (used == 0) {
       if (!wait_lead<A>(base, e, ep4, fault, spins)) break;
       continue;

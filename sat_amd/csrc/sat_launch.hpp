@@ -66,6 +66,7 @@ inline Geometry geom_of() {
   g.tw = Op::TW;
   g.vwords = Op::VWORDS;
   g.hwords = Op::HWORDS;
+  g.gwords = Op::GWORDS;
   g.threads = Op::NT;
   return g;
 }
@@ -95,7 +96,9 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   if (ntiles == 0) return SAT_OK;
   if (ntiles > 0x7fffffffull) return SAT_ERR_INVALID;
   const uint64_t vw = ntiles * Op::VWORDS, hw = ntiles * Op::HWORDS;
-  const uint64_t need = satdev::WS_HEADER_BYTES + (vw + hw) * 8;
+  const uint32_t vgroup = Op::GROUPS ? vgroup_len(nTx, nTy) : 0u;
+  const uint64_t gw = vgroup ? (uint64_t)L.planes * ((nTy + vgroup - 1) / vgroup) * nTx * Op::GWORDS : 0u;
+  const uint64_t need = satdev::WS_HEADER_BYTES + (vw + hw + gw) * 8;
   if (ws == nullptr) return SAT_ERR_NULL;
   if (ws_bytes < need) return SAT_ERR_WORKSPACE;
   uint64_t* carries = (uint64_t*)((char*)ws + satdev::WS_HEADER_BYTES);
@@ -115,6 +118,7 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   satdev::TileOrder order;
   order.nx = (uint32_t)device_xcd_count();
   order.run = (order.nx > 1 && nTx % order.nx == 0) ? run_length((uint32_t)(nTx / order.nx)) : 0u;
+  order.vgroup = vgroup;
   hipLaunchKernelGGL((satdev::tiles_kernel<Op, WPE>), dim3((uint32_t)ntiles), dim3(Op::NT), 0, stream, g, (uint32_t)tpp,
                      L.in_plane, L.out_plane, L.epoch, (const satdev::WsHeader*)ws, order);
   return hipGetLastError() == hipSuccess ? SAT_OK : SAT_ERR_HIP;

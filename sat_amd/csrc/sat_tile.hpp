@@ -45,6 +45,20 @@ constexpr int LB_WINDOW_V = 1;
 // 4096^2 and 8192^2; 8 was slower again), while 4-byte payloads lost with 4.
 template <typename A> constexpr int lb_window_h() { return sizeof(A) == 8 ? 4 : LB_WINDOW; }
 
+// Second carry level of the vertical chains of tall images: the tiles of a
+// tile column are grouped in runs of `len` tile rows, and the last tile of a
+// run publishes the run's column sums (AGG) and its inclusive column sums
+// (INC) in `carry` (one entry per run, tile column and column, planes after
+// each other). A tile's vertical look-back walks its own run's tiles (one
+// window) and then the earlier runs' words, so a chain of n tiles takes about
+// len + n / len steps instead of up to n. len == 0: one level. Only 4-byte
+// payloads take it: the extra code made the 8-byte kernels spill registers
+// (f64 8192^2 246 -> 254 us with no run words in use).
+struct VGroups {
+  uint64_t* carry = nullptr;
+  uint32_t len = 0;
+};
+
 template <bool C, typename T, typename F> struct cond { using type = T; };
 template <typename T, typename F> struct cond<false, T, F> { using type = F; };
 
@@ -81,6 +95,9 @@ struct TileOp {
   static constexpr int64_t VS = TW + (CORNER ? 8 : 0);
   static constexpr int64_t VWORDS = DO_COL ? VS * Carry<A>::WORDS : 0;  // per tile
   static constexpr int64_t HWORDS = DO_ROW ? (int64_t)TH * Carry<A>::WORDS : 0;  // per tile
+  // run words (VGroups) for tiles whose vertical lanes follow one column each
+  static constexpr bool GROUPS = DO_COL && CPV == 1 && Carry<A>::WORDS == 1;
+  static constexpr int64_t GWORDS = GROUPS ? (int64_t)TW * Carry<A>::WORDS : 0;  // per run and tile column
   static_assert(!(TOUT && (REV || DO_COL)), "transposed store is a row-scan pass only");
   // the horizontal look-back wave follows KH rows per lane: rows l, l+64, ...
   // (lanes of the last chain past the tile's rows take no part)
@@ -137,7 +154,8 @@ struct TileOp {
     return k.py0 + TH <= (int64_t)g.H && k.px0 + TW <= (int64_t)g.W;
   }
 
-  __device__ static __forceinline__ void run(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm) {
+  __device__ static __forceinline__ void run(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm,
+                                             const VGroups& vg = VGroups{}) {
     const Coords k = coords(g, t);
     const TSrc* __restrict__ in = (const TSrc*)g.in;
     // ---- load (each wave: RPT coalesced row segments of 256 elements) ----
@@ -190,7 +208,7 @@ struct TileOp {
         for (int c = 0; c < 4; ++c) v[r][c] = REV ? x[3 - c] : x[c];
       }
     }
-    finish(g, t, ep4, sm, k, v);
+    finish(g, t, ep4, sm, k, v, vg);
   }
 
   // The tile's column sums, per wave in sm.ct, summed over the waves: the
@@ -220,8 +238,33 @@ struct TileOp {
   // prefixed over the tile's rows for the 2-D SAT, whose total is the corner
   // slot's aggregate), the corner on wave CWAVE into sm.corner; each
   // publishes its inclusive words. Ends after a barrier.
+  // Vertical look-back of a one-column lane over two carry levels (VGroups):
+  // acc = column sum of every tile above. The last tile of a run publishes
+  // the run's aggregate before it waits on earlier runs, and the run's
+  // inclusive sums after.
+  __device__ static __forceinline__ A group_look_back(const ImgArgs& g, const VGroups& vg, uint32_t t, uint32_t I,
+                                                      uint32_t J, uint32_t ep4, A colsum) {
+    const int tid = threadIdx.x;
+    const uint32_t gi = I / vg.len, m = I - gi * vg.len;  // run index, predecessors inside the run
+    const int64_t gstep = (int64_t)g.nTx * TW;
+    const int64_t ge = ((int64_t)gi * g.nTx + J) * TW + tid;
+    A acc = zero<A>();
+    bool inc = false;
+    if (m > 0)
+      acc = look_back<A, 8>(g.vcarry, (int64_t)(t - g.nTx) * VS + tid, (int64_t)g.nTx * VS, m, ep4, g.fault, &inc);
+    const bool last = m == vg.len - 1;
+    if (!inc) {
+      // every tile of the run above this one was an aggregate (run 0 always
+      // ends at the inclusive first tile row)
+      if (last) Carry<A>::put(vg.carry, ge, acc + colsum, ep4 | ST_AGG);
+      if (gi > 0) acc += look_back<A, 4>(vg.carry, ge - gstep, gstep, gi, ep4, g.fault);
+    }
+    if (last) Carry<A>::put(vg.carry, ge, acc + colsum, ep4 | ST_INC);
+    return acc;
+  }
+
   __device__ static __forceinline__ void lookbacks(const ImgArgs& g, uint32_t t, uint32_t I, uint32_t J, uint32_t ep4,
-                                                   Smem& sm, const A (&colsum)[CPV]) {
+                                                   Smem& sm, const A (&colsum)[CPV], const VGroups& vg = VGroups{}) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if constexpr (DO_COL) {
       if (w < NVW) {
@@ -232,10 +275,17 @@ struct TileOp {
           int64_t e[CPV];
 #pragma unroll
           for (int q = 0; q < CPV; ++q) e[q] = (int64_t)(t - g.nTx) * VS + q * 64 * NVW + tid;
+          bool grouped = false;
+          if constexpr (GROUPS) {
+            if (vg.len) {
+              acc[0] = group_look_back(g, vg, t, I, J, ep4, colsum[0]);
+              grouped = true;
+            }
+          }
           // the two column chains of a lane are read one after the other:
           // reading both at once measured 1-3 % slower on 64-row f32 and
           // u32 tiles (their look-backs are not what bounds them)
-          look_back_k<A, LB_WINDOW_V, CPV, false>(g.vcarry, e, (int64_t)g.nTx * VS, I, ep4, g.fault, acc);
+          if (!grouped) look_back_k<A, LB_WINDOW_V, CPV, false>(g.vcarry, e, (int64_t)g.nTx * VS, I, ep4, g.fault, acc);
 #pragma unroll
           for (int q = 0; q < CPV; ++q)
             Carry<A>::put(g.vcarry, (int64_t)t * VS + q * 64 * NVW + tid, acc[q] + colsum[q], ep4 | ST_INC);
@@ -355,7 +405,7 @@ struct TileOp {
   // Everything after the load: scans, look-backs, store. `v` holds the tile
   // in logical order (mirrored when REV).
   __device__ static __forceinline__ void finish(const ImgArgs& g, uint32_t t, uint32_t ep4, Smem& sm,
-                                                const Coords& k, A (&v)[RPT][4]) {
+                                                const Coords& k, A (&v)[RPT][4], const VGroups& vg = VGroups{}) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t I = k.I, J = k.J;
     const int64_t py0 = k.py0, px0 = k.px0, pcol = k.pcol;
@@ -415,7 +465,7 @@ struct TileOp {
       __syncthreads();
     }
 
-    lookbacks(g, t, I, J, ep4, sm, colsum);
+    lookbacks(g, t, I, J, ep4, sm, colsum, vg);
 
     // ---- per-column and per-row additions ----
     A colAdd[4];
@@ -533,6 +583,8 @@ struct PackedTileOp {
   using Smem = typename Base::Smem;
   static constexpr int TW = Base::TW, TH = Base::TH, NT = Base::NT;
   static constexpr int64_t VWORDS = Base::VWORDS, HWORDS = Base::HWORDS;
+  static constexpr bool GROUPS = false;
+  static constexpr int64_t GWORDS = 0;
   static constexpr bool SIGNED = (TSrc)(-1) < (TSrc)0;
   static_assert(sizeof(TSrc) == 1 && sizeof(A) == 4, "1-byte inputs, 4-byte outputs");
   static_assert(RPT <= 128, "16-bit column partial sums");
@@ -688,6 +740,8 @@ struct SerialTileOp {
   static constexpr int RPL = NT / LPR; // rows loaded per pass
   static constexpr int64_t VWORDS = 0;
   static constexpr int64_t HWORDS = (int64_t)TH * Carry<A>::WORDS;
+  static constexpr bool GROUPS = false;
+  static constexpr int64_t GWORDS = 0;
   static_assert(TW % 32 == 0 && NT % LPR == 0 && TH % RPL == 0, "tile shape");
 
   struct Smem {
@@ -832,8 +886,9 @@ struct SerialTileOp {
 // t % tpp of that plane (planes stack vertically in the tile grid).
 // ---------------------------------------------------------------------------
 struct TileOrder {
-  uint32_t nx;   // XCDs the workgroups are dealt over
-  uint32_t run;  // tile columns of one run (0: identity order); divides nTx / nx
+  uint32_t nx;      // XCDs the workgroups are dealt over
+  uint32_t run;     // tile columns of one run (0: identity order); divides nTx / nx
+  uint32_t vgroup;  // tile rows per run of the second vertical carry level (0: none)
 };
 
 // Runs of `run` tile columns are dealt to the XCDs round-robin (run q of a
@@ -865,7 +920,18 @@ __global__ void __launch_bounds__(Op::NT) __attribute__((amdgpu_waves_per_eu(WPE
     gp.vcarry = g.vcarry + (int64_t)p * tpp * Op::VWORDS;
     gp.hcarry = g.hcarry + (int64_t)p * tpp * Op::HWORDS;
   }
-  Op::run(gp, lt, ep4, sm);
+  if constexpr (Op::GROUPS) {
+    VGroups vg;
+    if (order.vgroup) {
+      // run words follow the horizontal words of all planes, plane by plane
+      const uint32_t runs = (g.nTy + order.vgroup - 1) / order.vgroup;
+      vg.carry = g.hcarry + (int64_t)gridDim.x * Op::HWORDS + (int64_t)p * runs * g.nTx * Op::GWORDS;
+      vg.len = order.vgroup;
+    }
+    Op::run(gp, lt, ep4, sm, vg);
+  } else {
+    Op::run(gp, lt, ep4, sm);
+  }
 }
 
 }  // namespace satdev

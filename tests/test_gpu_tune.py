@@ -157,3 +157,51 @@ def test_packed_all_255_wraps(dev, sa, cfg):
     xs = torch.full((H, W), -128, dtype=torch.int8, device=dev)
     ys = sa.integral(xs, out_dtype=torch.int32).to(torch.int64) & 0xFFFFFFFF
     assert torch.equal(ys, (-128 * i1 * j1) & 0xFFFFFFFF)
+
+
+# Images of one tile column and at least 64 tile rows take a second vertical
+# carry level (runs of 16 tile rows) for 4-byte payloads in the
+# configurations whose vertical lanes follow one column each (1, 2, 3): tall
+# shapes with partial last runs and tiles, one-pixel-wide images, planes, the
+# reverse SAT and the column pass of algorithm 2; 8-byte outputs and two tile
+# columns (one carry level) alongside.
+TALL = [((5197, 256), 2), ((5197, 1), 2), ((9000, 255), 1), ((4100, 130), 3), ((16384 + 77, 200), 1),
+        ((5197, 300), 2)]
+
+
+@pytest.mark.parametrize("tin,tout", [("u32", "u32"), ("f32", "f32"), ("f64", "f64"), ("u8", "u32")])
+def test_tall_two_level_chains(O, dev, sa, tin, tout):
+    L = sa.lib()
+    ob = 8 if tout == "f64" else 4
+    for k, ((h, w), cfg) in enumerate(TALL):
+        say(f"[case] two-level chains {tin}->{tout} {h}x{w} cfg {cfg}")
+        force(L, IN_BYTES[tin], ob, w, h, 1, cfg)
+        x = O.gen(KIND[tin], (h, w), 6100 + k)
+        y = sa.integral(to_dev(x, dev), out_dtype=TD[tout])
+        assert_sat_close(O, x, to_np(y), tout)
+    sa.check_faults()
+
+
+def test_tall_two_level_planes_reverse_and_column_pass(O, dev, sa):
+    L = sa.lib()
+    P, h, w = 3, 5197, 250
+    say("[case] two-level chains: planes, reverse SAT, algorithm 2")
+    x = O.gen("u32_20", (P, h, w), 6200)
+    force(L, 4, 4, w, h, P, 2)
+    y = to_np(sa.integral(to_dev(x, dev)))
+    for p in range(P):
+        assert np.array_equal(y[p], O.serial(x[p], np.uint32))
+    for kind, dt, cfg in (("f32", "f32", 1), ("f64n", "f64", 1)):
+        xf = O.gen(kind, (P, h, w), 6201)
+        force(L, IN_BYTES[dt], IN_BYTES[dt], w, h, P, cfg)
+        r = to_np(sa.integral_reverse(torch.from_numpy(xf).to(dev)))
+        for p in range(P):
+            flip = np.ascontiguousarray(xf[p, ::-1, ::-1])
+            assert_sat_close(O, flip, np.ascontiguousarray(r[p, ::-1, ::-1]), dt)
+    xr = O.gen("u32_20", (h, w), 6203)
+    force(L, 4, 4, w, h, 1, 2)
+    r = to_np(sa.integral_reverse(to_dev(xr, dev)))
+    assert np.array_equal(r, O.reverse(xr))
+    xu = O.gen("u32_20", (9000, 256), 6202)
+    assert np.array_equal(to_np(sa.integral(to_dev(xu, dev), alg=2)), O.serial(xu, np.uint32))
+    sa.check_faults()
