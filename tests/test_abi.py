@@ -113,6 +113,14 @@ def test_workspace_sizes(L):
     # f64: 32x256 tiles, two words per carry
     tiles = (4096 // 32) * (4096 // 256)
     assert L.sat_integral_workspace_bytes(3, 7, 7, 4096, 4096) == 256 + tiles * (264 + 32) * 16
+    # one tile column of 64-row tiles (configurations 2 and 3, one column per
+    # vertical lane) adds the second vertical carry level: one word per column
+    # for every run of 16 tile rows
+    tiles = 65536 // 64
+    assert L.sat_integral_workspace_bytes(3, 6, 6, 256, 65536) == 256 + (tiles * (264 + 64) + (tiles // 16) * 256) * 8
+    # two tile columns: one level
+    tiles = (65536 // 64) * 2
+    assert L.sat_integral_workspace_bytes(3, 6, 6, 512, 65536) == 256 + tiles * (264 + 64) * 8
     # two-pass algorithms add a temporary of the output type
     assert L.sat_integral_workspace_bytes(2, 6, 6, 1000, 1000) >= 1024 * 1000 * 4
     assert L.sat_integral_workspace_bytes(0, 0, 4, 1000, 1000) >= 1024 * 1000 * 4

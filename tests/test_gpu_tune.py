@@ -205,3 +205,30 @@ def test_tall_two_level_planes_reverse_and_column_pass(O, dev, sa):
     xu = O.gen("u32_20", (9000, 256), 6202)
     assert np.array_equal(to_np(sa.integral(to_dev(xu, dev), alg=2)), O.serial(xu, np.uint32))
     sa.check_faults()
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+def test_tall_caller_workspace_exact_size(O, dev, sa, cfg):
+    """A caller workspace of exactly sat_integral_workspace_bytes, garbage
+    after its zeroed header, serves a one-column image in every
+    configuration (with and without the second carry level), twice."""
+    import ctypes as C
+
+    say(f"[case] tall image, caller workspace of the queried size, cfg {cfg}")
+    L = sa.lib()
+    h, w = 20000 + 3, 256
+    force(L, 4, 4, w, h, 1, cfg)
+    x = O.gen("u32_20", (h, w), 6300 + cfg)
+    xd = to_dev(x, dev)
+    y = torch.empty((h, w), dtype=torch.uint32, device=dev)
+    nb = L.sat_integral_workspace_bytes(3, sa.U32, sa.U32, w, h)
+    ws = torch.full((nb,), 0xA5, dtype=torch.uint8, device=dev)
+    ws[:256].zero_()
+    st = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        y.view(torch.uint8).zero_()
+        assert L.sat_integral(3, 0, sa.U32, sa.U32, xd.data_ptr(), w, y.data_ptr(), w, w, h,
+                              C.c_void_p(ws.data_ptr()), nb, C.c_void_p(st)) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(to_np(y), O.serial(x, np.uint32))
+    assert L.sat_workspace_fault(C.c_void_p(ws.data_ptr()), C.c_void_p(st)) == 0
