@@ -191,7 +191,7 @@ def test_tall_two_level_planes_reverse_and_column_pass(O, dev, sa):
     y = to_np(sa.integral(to_dev(x, dev)))
     for p in range(P):
         assert np.array_equal(y[p], O.serial(x[p], np.uint32))
-    for kind, dt, cfg in (("f32", "f32", 1), ("f64n", "f64", 1)):
+    for kind, dt, cfg in (("f32", "f32", 2), ("f32", "f32", 1), ("f64n", "f64", 1)):
         xf = O.gen(kind, (P, h, w), 6201)
         force(L, IN_BYTES[dt], IN_BYTES[dt], w, h, P, cfg)
         r = to_np(sa.integral_reverse(torch.from_numpy(xf).to(dev)))
