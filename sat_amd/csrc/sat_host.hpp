@@ -35,7 +35,8 @@ struct Geometry {
 // 16 let a look-back skip a run per step: f32 65536 x 256 (512 tiles of 128
 // rows) 31.7 -> 29.3 us, 16384 x 256 17.4 -> 16.6 us, u32 65536 x 256 31.7 ->
 // 29.5 us (runs of 8 or 32 and other windows: equal or slower). With two tile
-// columns it measured slower (f32 65536 x 512: 56.5 -> 60.0 us).
+// columns it measured slower (f32 65536 x 512: 56.5 -> 60.0 us). Launches
+// with more tiles than resident workgroups do not use it (launch_op).
 inline uint32_t vgroup_len(uint64_t nTx, uint64_t nTy) { return (nTx == 1 && nTy >= 64) ? 16u : 0u; }
 // Tile configurations of the one-pass families (FAM_SAT, FAM_SAT_REV), cfg in
 // [0, TILE_CFGS), and of the batch kernel, cfg in [0, BATCH_CFGS); other

@@ -96,8 +96,13 @@ inline sat_status launch_op(const Launch& L, void* ws, size_t ws_bytes, hipStrea
   if (ntiles == 0) return SAT_OK;
   if (ntiles > 0x7fffffffull) return SAT_ERR_INVALID;
   const uint64_t vw = ntiles * Op::VWORDS, hw = ntiles * Op::HWORDS;
-  const uint32_t vgroup = Op::GROUPS ? vgroup_len(nTx, nTy) : 0u;
+  uint32_t vgroup = Op::GROUPS ? vgroup_len(nTx, nTy) : 0u;
   const uint64_t gw = vgroup ? (uint64_t)L.planes * ((nTy + vgroup - 1) / vgroup) * nTx * Op::GWORDS : 0u;
+  // the run words are laid out whenever the shape qualifies (workspace sizes
+  // depend on the shape only) but used only when every tile of the launch is
+  // resident at once: with later generations of tiles they measured slower
+  // (f32 262144 x 256 105 -> 114 us, 1048576 x 256 359 -> 445 us)
+  if (vgroup && ntiles > (uint64_t)device_cu_count() * (4 * WPE / (Op::NT / 64))) vgroup = 0;
   const uint64_t need = satdev::WS_HEADER_BYTES + (vw + hw + gw) * 8;
   if (ws == nullptr) return SAT_ERR_NULL;
   if (ws_bytes < need) return SAT_ERR_WORKSPACE;

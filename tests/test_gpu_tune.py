@@ -207,16 +207,20 @@ def test_tall_two_level_planes_reverse_and_column_pass(O, dev, sa):
     sa.check_faults()
 
 
+@pytest.mark.parametrize("h", [20000 + 3, 131072 + 5])
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3])
-def test_tall_caller_workspace_exact_size(O, dev, sa, cfg):
+def test_tall_caller_workspace_exact_size(O, dev, sa, cfg, h):
     """A caller workspace of exactly sat_integral_workspace_bytes, garbage
     after its zeroed header, serves a one-column image in every
-    configuration (with and without the second carry level), twice."""
+    configuration, twice: with the second carry level (20003 rows in
+    configurations 1-3) and without it (configuration 0, and 131077 rows,
+    more tiles than resident workgroups, whose workspace still has room for
+    the run words)."""
     import ctypes as C
 
-    say(f"[case] tall image, caller workspace of the queried size, cfg {cfg}")
+    say(f"[case] tall image {h} x 256, caller workspace of the queried size, cfg {cfg}")
     L = sa.lib()
-    h, w = 20000 + 3, 256
+    w = 256
     force(L, 4, 4, w, h, 1, cfg)
     x = O.gen("u32_20", (h, w), 6300 + cfg)
     xd = to_dev(x, dev)
