@@ -344,6 +344,44 @@ __device__ __forceinline__ A look_back(const uint64_t* base, int64_t e, int64_t 
   return acc;
 }
 
+// Two look-back chains of one-word payloads in adjacent words (entries e and
+// e + 1 of every predecessor), published together by one 16-byte store: each
+// round reads both words of one predecessor with one 16-byte load. Both
+// chains stay on the same predecessor; a chain that met an inclusive word
+// ignores the rest. A pair with an unpublished word of an unfinished chain is
+// read again once wait_lead has seen that word published.
+template <typename A>
+__device__ __forceinline__ void look_back_pair(const uint64_t* base, int64_t e, int64_t step, uint32_t n,
+                                               uint32_t ep4, const Fault& fault, A (&acc)[2]) {
+  static_assert(Carry<A>::WORDS == 1, "one-word payloads");
+  acc[0] = zero<A>();
+  acc[1] = zero<A>();
+  bool fin0 = n == 0, fin1 = n == 0;
+  uint32_t spins = 0;
+  while (!(fin0 && fin1)) {
+    const u64x2 w = ld_pair(base + e);
+    typename Carry<A>::Raw r0, r1;
+    r0.w[0] = w.x;
+    r1.w[0] = w.y;
+    A v0, v1;
+    const uint32_t s0 = Carry<A>::decode(r0, ep4, v0), s1 = Carry<A>::decode(r1, ep4, v1);
+    if ((!fin0 && s0 == 0) || (!fin1 && s1 == 0)) {
+      if (!wait_lead<A>(base, (!fin0 && s0 == 0) ? e : e + 1, ep4, fault, spins)) break;
+      continue;
+    }
+    if (!fin0) {
+      acc[0] += v0;
+      fin0 = s0 == ST_INC;
+    }
+    if (!fin1) {
+      acc[1] += v1;
+      fin1 = s1 == ST_INC;
+    }
+    if (--n == 0) break;
+    e -= step;
+  }
+}
+
 // K independent look-back chains per lane over the same predecessors (for
 // example two columns of one tile). With BATCH every round issues the windows
 // of all K chains before it uses any, so the chains share one memory round

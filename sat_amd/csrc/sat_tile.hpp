@@ -86,6 +86,12 @@ struct TileOp {
   // pass), the horizontal and corner look-backs on the last wave
   static constexpr int NVW = !DO_COL ? 1 : (DO_ROW ? NW / 2 : NW);
   static constexpr int CPV = TW / (64 * NVW);  // columns per vertical lane
+  // two adjacent columns per vertical lane, their words published and read as
+  // one 16-byte pair, so both chains take one round trip per predecessor
+  // (f32 8192^2 107.3 -> 106.2 us, u32 -0.6 %, u8 -> u32 8192^2 -1.7 %; the
+  // columns of a lane were 128 apart, read one chain after the other)
+  static constexpr bool VPAIR = CPV == 2 && Carry<A>::WORDS == 1;
+  __device__ static __forceinline__ int vcol(int q, int tid) { return VPAIR ? 2 * tid + q : q * 64 * NVW + tid; }
   static constexpr int HWAVE = NW - 1;
   // corner look-back: its own wave when there is one to spare
   static constexpr int CWAVE = (NW >= 3 && NVW <= NW - 2) ? NW - 2 : HWAVE;
@@ -222,12 +228,19 @@ struct TileOp {
     if (w < NVW) {
 #pragma unroll
       for (int q = 0; q < CPV; ++q) {
-        const int j = q * 64 * NVW + tid;
+        const int j = vcol(q, tid);
         A cs = sm.ct[0][j];
 #pragma unroll
         for (int u = 1; u < NW; ++u) cs += sm.ct[u][j];
         colsum[q] = cs;
-        Carry<A>::put(g.vcarry, (int64_t)t * VS + j, cs, ep4 | (I == 0 ? ST_INC : ST_AGG));
+        if constexpr (!VPAIR) Carry<A>::put(g.vcarry, (int64_t)t * VS + j, cs, ep4 | (I == 0 ? ST_INC : ST_AGG));
+      }
+      if constexpr (VPAIR) {
+        const uint64_t tag = (uint64_t)(ep4 | (I == 0 ? ST_INC : ST_AGG)) << 32;
+        u64x2 pw;
+        pw.x = tag | __builtin_bit_cast(uint32_t, colsum[0]);
+        pw.y = tag | __builtin_bit_cast(uint32_t, colsum[1]);
+        st_pair(g.vcarry + (int64_t)t * VS + 2 * tid, pw);
       }
     }
   }
@@ -274,7 +287,7 @@ struct TileOp {
         if (I > 0) {
           int64_t e[CPV];
 #pragma unroll
-          for (int q = 0; q < CPV; ++q) e[q] = (int64_t)(t - g.nTx) * VS + q * 64 * NVW + tid;
+          for (int q = 0; q < CPV; ++q) e[q] = (int64_t)(t - g.nTx) * VS + vcol(q, tid);
           bool grouped = false;
           if constexpr (GROUPS) {
             if (vg.len) {
@@ -282,16 +295,27 @@ struct TileOp {
               grouped = true;
             }
           }
-          // the two column chains of a lane are read one after the other:
-          // reading both at once measured 1-3 % slower on 64-row f32 and
-          // u32 tiles (their look-backs are not what bounds them)
-          if (!grouped) look_back_k<A, LB_WINDOW_V, CPV, false>(g.vcarry, e, (int64_t)g.nTx * VS, I, ep4, g.fault, acc);
+          // two one-word column chains of a lane: adjacent words, one 16-byte
+          // load per predecessor for both. Otherwise the chains are read one
+          // after the other: two separate loads per round for both at once
+          // measured 1-3 % slower on 64-row f32 and u32 tiles
+          if constexpr (VPAIR) {
+            look_back_pair<A>(g.vcarry, e[0], (int64_t)g.nTx * VS, I, ep4, g.fault, acc);
+            const uint64_t tag = (uint64_t)(ep4 | ST_INC) << 32;
+            u64x2 pw;
+            pw.x = tag | __builtin_bit_cast(uint32_t, acc[0] + colsum[0]);
+            pw.y = tag | __builtin_bit_cast(uint32_t, acc[1] + colsum[1]);
+            st_pair(g.vcarry + (int64_t)t * VS + 2 * tid, pw);
+          } else {
+            if (!grouped)
+              look_back_k<A, LB_WINDOW_V, CPV, false>(g.vcarry, e, (int64_t)g.nTx * VS, I, ep4, g.fault, acc);
 #pragma unroll
-          for (int q = 0; q < CPV; ++q)
-            Carry<A>::put(g.vcarry, (int64_t)t * VS + q * 64 * NVW + tid, acc[q] + colsum[q], ep4 | ST_INC);
+            for (int q = 0; q < CPV; ++q)
+              Carry<A>::put(g.vcarry, (int64_t)t * VS + vcol(q, tid), acc[q] + colsum[q], ep4 | ST_INC);
+          }
         }
 #pragma unroll
-        for (int q = 0; q < CPV; ++q) sm.cab[q * 64 * NVW + tid] = acc[q];
+        for (int q = 0; q < CPV; ++q) sm.cab[vcol(q, tid)] = acc[q];
       }
     }
     if constexpr (DO_ROW) {
