@@ -121,3 +121,29 @@ def test_f32_96_planes_1024(O, dev, sa):
     y = sa.integral(torch.from_numpy(x).to(dev)).cpu().numpy()
     for n, c in ((0, 0), (7, 1), (19, 2), (31, 2)):
         assert_sat_close(O, np.ascontiguousarray(x[n, c]), y[n, c], "f32")
+
+
+@pytest.mark.parametrize("cfg", [0, 5])
+def test_u8_u32_beyond_2_32_pixels(dev, sa, cfg):
+    """65539 x 65792 u8 -> u32 (4.31e9 pixels, a 16 GiB table): element
+    offsets past 2^32 on the vector path, in the 64-row tiles and in the
+    packed 256-row tiles, checked exactly through the 2-D difference."""
+    H, W = 65536 + 3, 65536 + 256
+    say(f"[full] u8->u32 {H}x{W} ({H * W / 2**32:.2f} x 2^32 pixels), cfg {cfg}")
+    L = sa.lib()
+    import ctypes as C
+
+    cx, cy, cp = C.c_int(), C.c_int(), C.c_int()
+    L.sat_tune_class(W, H, 1, C.byref(cx), C.byref(cy), C.byref(cp))
+    before = L.sat_tune_get(1, 4, cx.value, cy.value, cp.value)
+    assert L.sat_tune_set(1, 4, cx.value, cy.value, cp.value, cfg) == 0
+    try:
+        x = gen_int(dev, "u8", (H, W), 20260107)
+        y = sa.integral(x, out_dtype=torch.uint32)
+        torch.cuda.synchronize()
+        assert_inverts(y, x, chunk=1024)
+        sa.check_faults()
+    finally:
+        L.sat_tune_set(1, 4, cx.value, cy.value, cp.value, before)
+        x = y = None
+        torch.cuda.empty_cache()
