@@ -236,3 +236,21 @@ def test_tall_caller_workspace_exact_size(O, dev, sa, cfg, h):
         torch.cuda.synchronize()
         assert np.array_equal(to_np(y), O.serial(x, np.uint32))
     assert L.sat_workspace_fault(C.c_void_p(ws.data_ptr()), C.c_void_p(st)) == 0
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 5])
+@pytest.mark.parametrize("tin,tout", [("u8", "u32"), ("f32", "f32"), ("f64", "f64")])
+def test_one_row_and_one_column_extremes(O, dev, sa, cfg, tin, tout):
+    """A single row of a million pixels (one tile row: a row chain of
+    thousands of tiles, partial tile rows) and a single column of a million
+    (one-pixel-wide tiles, a column chain of thousands of tiles in
+    generations), in several configurations."""
+    L = sa.lib()
+    ob = 8 if tout == "f64" else 4
+    for k, (h, w) in enumerate([(1, 1000003), (1000003, 1), (3, 333333)]):
+        say(f"[case] extreme shape {h}x{w} {tin}->{tout} cfg {cfg}")
+        force(L, IN_BYTES[tin], ob, w, h, 1, cfg)
+        x = O.gen(KIND[tin], (h, w), 6400 + k)
+        y = sa.integral(to_dev(x, dev), out_dtype=TD[tout])
+        assert_sat_close(O, x, to_np(y), tout)
+    sa.check_faults()
