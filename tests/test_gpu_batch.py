@@ -228,3 +228,15 @@ def test_batch_device_epoch_wrap(O, dev, sa, persistent):
         plan.check_faults()
     finally:
         L.sat_batch_persistent_set(before)
+
+
+def test_batch_many_tiny_images(O, dev, sa):
+    """3000 images of 1 to 40 pixels a side (edge tiles only, many more
+    images than workgroups per XCD), mixed u8 -> u32 / f32, one launch."""
+    say("[case] batch of 3000 tiny images")
+    imgs = make_batch(O, dev, 3000, 1, 40, 41)
+    plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+    plan.run()
+    torch.cuda.synchronize()
+    check_batch(O, imgs)
+    plan.check_faults()
