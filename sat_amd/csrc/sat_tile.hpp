@@ -344,8 +344,13 @@ struct TileOp {
             }
             // two row chains per lane are read together (f32 16384^2 with
             // 128-row tiles: 383 -> 362 us); four (256-row packed tiles)
-            // one after the other, which measured 1-4 % faster there
-            look_back_k<A, lb_window_h<A>(), KH, (KH <= 2)>(g.hcarry, e, TH, J, ep4, g.fault, rab, skip);
+            // one after the other, which measured 1-4 % faster there. Tiles
+            // of 1-byte inputs into 4-byte outputs read one predecessor per
+            // round: u8 -> u32
+            // 16384^2 273.2 -> 268.5 us and 8192^2 -1.7 % (2 for the rest:
+            // f32 16384^2 +0.7 % and 256 x 65536 +2 % with 1)
+            look_back_k<A, (sizeof(TSrc) == 1 && sizeof(A) == 4 ? 1 : lb_window_h<A>()), KH, (KH <= 2)>(
+                g.hcarry, e, TH, J, ep4, g.fault, rab, skip);
 #pragma unroll
             for (int k = 0; k < KH; ++k)
               if (!skip[k])
