@@ -501,6 +501,15 @@ static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_
 // length. Each image's tiles stay row-major and in increasing block order, so
 // the queue is also safe for any other dealing of workgroups that starts them
 // in increasing b.
+//
+// An image larger than one XCD's share of the batch's bytes is not dealt
+// whole (the XCDs holding none would idle: 4 f32 8192^2 images ran at 48 % of
+// HBM peak): such images come first in the queue, their tiles row-major at
+// consecutive blocks 0, 1, 2, ... over all XCDs, and the dealt images follow
+// from the first whole round of blocks after them. A look-back of such an
+// image reads tiles of other XCDs, which have smaller block numbers. Plans
+// holding such images take the one-workgroup-per-item kernel: the persistent
+// one only ever waits within one XCD's list.
 // ---------------------------------------------------------------------------
 constexpr uint32_t QUEUE_EMPTY = 0xffffffffu;
 
@@ -510,6 +519,7 @@ struct BatchPlanInfo {
   uint32_t total_tiles = 0;
   uint32_t blocks = 0;  // queue items (tiles + empty items), items_per_xcd * nx
   uint32_t nx = 1;
+  uint32_t spread = 0;  // images laid over all XCDs
   uint64_t carry_words = 0;
   size_t items_off = 0, queue_off = 0, tickets_off = 0, total = 0;
   bool persistent = false;
@@ -523,11 +533,10 @@ static std::atomic<int> g_batch_persistent{0};
 static std::mutex g_batch_mu;
 static std::map<uintptr_t, BatchPlanInfo> g_batch;
 
-// Deal images to XCDs (see the layout comment): per-XCD lists of image ids,
-// balanced by bytes moved.
-static std::vector<std::vector<uint32_t>> deal_images(const std::vector<uint64_t>& bytes, int nx) {
-  std::vector<uint32_t> order(bytes.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
+// Deal images `ids` to XCDs (see the layout comment): per-XCD lists of image
+// ids, balanced by bytes moved.
+static std::vector<std::vector<uint32_t>> deal_images(const std::vector<uint64_t>& bytes,
+                                                      std::vector<uint32_t> order, int nx) {
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return bytes[a] > bytes[b]; });
   std::vector<std::vector<uint32_t>> lists(nx);
   std::vector<uint64_t> load(nx, 0);
@@ -562,15 +571,29 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
   if (tiles > 0x7fffffffull) return SAT_ERR_INVALID;
   const int nx = std::max(1, device_xcd_count());
   std::vector<uint64_t> bytes(count);
+  uint64_t total_bytes = 0;
   for (int i = 0; i < count; ++i)
-    bytes[i] = (uint64_t)im[i].width * im[i].height * (dtype_size(im[i].tin) + dtype_size(im[i].tout));
-  const auto lists = deal_images(bytes, nx);
+    total_bytes += (bytes[i] = (uint64_t)im[i].width * im[i].height * (dtype_size(im[i].tin) + dtype_size(im[i].tout)));
+  // images above one XCD's share are spread over all XCDs, the rest dealt
+  std::vector<uint32_t> spread, dealt;
+  uint64_t spread_tiles = 0;
+  for (int i = 0; i < count; ++i) {
+    if (nx > 1 && bytes[i] * (uint64_t)nx > total_bytes) {
+      spread.push_back((uint32_t)i);
+      spread_tiles += ntiles[i];
+    } else {
+      dealt.push_back((uint32_t)i);
+    }
+  }
+  const auto lists = deal_images(bytes, dealt, nx);
+  const uint64_t k0 = (spread_tiles + nx - 1) / nx;  // rounds of blocks the spread images take
   uint64_t longest = 0;
   for (int x = 0; x < nx; ++x) {
     uint64_t n = 0;
     for (uint32_t i : lists[x]) n += ntiles[i];
     longest = std::max(longest, n);
   }
+  longest += k0;
   const uint64_t blocks = longest * (uint64_t)nx;
   if (blocks > 0x7fffffffull) return SAT_ERR_INVALID;
   info->out_size = osz;
@@ -578,6 +601,7 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
   info->total_tiles = (uint32_t)tiles;
   info->blocks = (uint32_t)blocks;
   info->nx = (uint32_t)nx;
+  info->spread = (uint32_t)spread.size();
   info->carry_words = tiles * (uint64_t)(g.vwords + g.hwords);
   info->items_off = (size_t)round_up(satdev::WS_HEADER_BYTES + (int64_t)info->carry_words * 8, 256);
   info->queue_off = (size_t)round_up((int64_t)(info->items_off + sizeof(BatchItem) * (size_t)count), 256);
@@ -586,8 +610,14 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
   if (!items) return SAT_OK;
   items->resize(count);
   queue->assign(2 * blocks, QUEUE_EMPTY);
+  uint64_t pos = 0;
+  for (uint32_t i : spread)
+    for (uint64_t t = 0; t < ntiles[i]; ++t, ++pos) {
+      (*queue)[2 * pos] = i;
+      (*queue)[2 * pos + 1] = (uint32_t)t;
+    }
   for (int x = 0; x < nx; ++x) {
-    uint64_t k = 0, rows = 0;
+    uint64_t k = k0, rows = 0;
     for (uint32_t i : lists[x]) rows = std::max<uint64_t>(rows, (im[i].height + g.th - 1) / g.th);
     for (uint64_t r = 0; r < rows; ++r)
       for (uint32_t i : lists[x]) {
@@ -968,7 +998,7 @@ int sat_batch_prepare(const sat_image* images, int count, void* workspace, size_
                      hipMemcpyHostToDevice, st) != hipSuccess)
     return SAT_ERR_HIP;
   if (hipStreamSynchronize(st) != hipSuccess) return SAT_ERR_HIP;
-  info.persistent = g_batch_persistent.load(std::memory_order_relaxed) != 0 && info.nx <= 256;
+  info.persistent = g_batch_persistent.load(std::memory_order_relaxed) != 0 && info.nx <= 256 && info.spread == 0;
   if (info.persistent) {
     // as many workgroups per XCD as are resident at once (more would only
     // wait), never more than the XCD's queue items

@@ -240,3 +240,49 @@ def test_batch_many_tiny_images(O, dev, sa):
     torch.cuda.synchronize()
     check_batch(O, imgs)
     plan.check_faults()
+
+
+def _spread_batch(O, dev):
+    """Three images that each hold more than an eighth of the batch's bytes
+    (laid over all XCDs) beside 24 small ones (dealt whole), mixed u8 -> u32 /
+    f32."""
+    imgs = []
+    for j, (h, w, kind) in enumerate(((1500, 1300, "f32"), (1100, 1700, "f32"), (2000, 1900, "u8"))):
+        x = O.gen(kind, (h, w), 7000 + j)
+        yd = torch.empty((h, w), dtype=torch.uint32 if kind == "u8" else torch.float32, device=dev)
+        imgs.append((kind, x, torch.from_numpy(x).to(dev), yd))
+    imgs += make_batch(O, dev, 24, 1, 260, 71)
+    return imgs
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_large_images_over_all_xcds(O, dev, sa, persistent):
+    """Images larger than one XCD's share of the bytes take consecutive queue
+    positions over all XCDs (their look-backs read tiles of other XCDs); the
+    persistent setting falls back to one workgroup per item for such a plan.
+    Every image against the oracle, twice, and a lone image and two images
+    (each above an XCD's share)."""
+    say(f"[case] batch with images spread over all XCDs, persistent={persistent}")
+    L = sa.lib()
+    before = L.sat_batch_persistent_get()
+    assert L.sat_batch_persistent_set(persistent) == 0
+    try:
+        imgs = _spread_batch(O, dev)
+        plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+        for rep in range(2):
+            for _, _, _, yd in imgs:
+                yd.zero_()
+            plan.run()
+            torch.cuda.synchronize()
+            check_batch(O, imgs)
+        plan.check_faults()
+        for sub in (imgs[:1], imgs[1:3]):
+            p = sa.BatchPlan([(xd, yd) for _, _, xd, yd in sub])
+            for _, _, _, yd in sub:
+                yd.zero_()
+            p.run()
+            torch.cuda.synchronize()
+            check_batch(O, sub)
+            p.check_faults()
+    finally:
+        L.sat_batch_persistent_set(before)
