@@ -248,8 +248,8 @@ int sat_batch_run(void* workspace, void* stream);
  * workgroup per queue item, which the hardware dispatches in queue order; 1
  * a persistent kernel, as many workgroups as are resident, each drawing the
  * next (image, tile) item of its XCD's queue from a ticket counter. Results
- * are the same; the default measured faster on MI355X. A batch holding an
- * image larger than one XCD's share of its bytes (such images are laid over
+ * are the same; the default measured faster on MI355X. A batch whose whole
+ * images would load the XCDs unevenly (its largest images are then laid over
  * all XCDs) always takes the first form. */
 int sat_batch_persistent_set(int on);
 int sat_batch_persistent_get(void);

@@ -1,6 +1,7 @@
 // Batched SAT: one launch walks a queue of (image, tile) items over images of
 // uneven sizes and mixed dtype pairs. The host deals whole images to XCDs
-// (images above one XCD's share of the bytes are laid over all XCDs instead)
+// (the largest are laid over all XCDs instead when whole images would load
+// the XCDs unevenly)
 // and keeps each image's tiles row-major and in increasing queue position, so
 // the look-back of a tile only ever waits for tiles of its own image that
 // precede it in the queue, on its own XCD for a dealt image.

@@ -502,9 +502,9 @@ static sat_status run_integral(SatPlan& P, void* ws, size_t ws_bytes, hipStream_
 // the queue is also safe for any other dealing of workgroups that starts them
 // in increasing b.
 //
-// An image larger than one XCD's share of the batch's bytes is not dealt
-// whole (the XCDs holding none would idle: 4 f32 8192^2 images ran at 48 % of
-// HBM peak): such images come first in the queue, their tiles row-major at
+// When whole images cannot load the XCDs evenly, the largest are not dealt
+// (the XCDs holding none would idle: 4 f32 8192^2 images ran at 48 % of HBM
+// peak): such images come first in the queue, their tiles row-major at
 // consecutive blocks 0, 1, 2, ... over all XCDs, and the dealt images follow
 // from the first whole round of blocks after them. A look-back of such an
 // image reads tiles of other XCDs, which have smaller block numbers. Plans
@@ -571,21 +571,33 @@ static sat_status batch_layout(const sat_image* im, int count, int cfg, BatchPla
   if (tiles > 0x7fffffffull) return SAT_ERR_INVALID;
   const int nx = std::max(1, device_xcd_count());
   std::vector<uint64_t> bytes(count);
-  uint64_t total_bytes = 0;
   for (int i = 0; i < count; ++i)
-    total_bytes += (bytes[i] = (uint64_t)im[i].width * im[i].height * (dtype_size(im[i].tin) + dtype_size(im[i].tout)));
-  // images above one XCD's share are spread over all XCDs, the rest dealt
-  std::vector<uint32_t> spread, dealt;
-  uint64_t spread_tiles = 0;
-  for (int i = 0; i < count; ++i) {
-    if (nx > 1 && bytes[i] * (uint64_t)nx > total_bytes) {
-      spread.push_back((uint32_t)i);
-      spread_tiles += ntiles[i];
-    } else {
-      dealt.push_back((uint32_t)i);
+    bytes[i] = (uint64_t)im[i].width * im[i].height * (dtype_size(im[i].tin) + dtype_size(im[i].tout));
+  // The s largest images are spread over all XCDs and the rest dealt whole,
+  // s the smallest count for which no XCD's dealt bytes exceed their mean by
+  // more than 2 % (past 64 tries every image is spread).
+  std::vector<uint32_t> by_size(count);
+  for (int i = 0; i < count; ++i) by_size[i] = (uint32_t)i;
+  std::stable_sort(by_size.begin(), by_size.end(), [&](uint32_t a, uint32_t b) { return bytes[a] > bytes[b]; });
+  size_t ns = 0;
+  std::vector<std::vector<uint32_t>> lists;
+  for (;; ++ns) {
+    if (nx > 1 && ns > 64) ns = (size_t)count;
+    const std::vector<uint32_t> rest(by_size.begin() + ns, by_size.end());
+    lists = deal_images(bytes, rest, nx);
+    if (nx == 1 || ns == (size_t)count) break;
+    uint64_t sum = 0, top = 0;
+    for (int x = 0; x < nx; ++x) {
+      uint64_t b = 0;
+      for (uint32_t i : lists[x]) b += bytes[i];
+      sum += b;
+      top = std::max(top, b);
     }
+    if (top * (uint64_t)nx * 50 <= sum * 51) break;
   }
-  const auto lists = deal_images(bytes, dealt, nx);
+  const std::vector<uint32_t> spread(by_size.begin(), by_size.begin() + ns);
+  uint64_t spread_tiles = 0;
+  for (uint32_t i : spread) spread_tiles += ntiles[i];
   const uint64_t k0 = (spread_tiles + nx - 1) / nx;  // rounds of blocks the spread images take
   uint64_t longest = 0;
   for (int x = 0; x < nx; ++x) {

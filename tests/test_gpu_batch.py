@@ -286,3 +286,21 @@ def test_batch_large_images_over_all_xcds(O, dev, sa, persistent):
             p.check_faults()
     finally:
         L.sat_batch_persistent_set(before)
+
+
+def test_batch_equal_images_one_more_than_xcds(O, dev, sa):
+    """Nine images of one size: dealt whole, one XCD would hold two; the
+    largest are laid over all XCDs until the rest deal evenly (mixed spread
+    and dealt images in one queue). Every image against the oracle."""
+    say("[case] batch of 9 equal images")
+    imgs = []
+    for j in range(9):
+        kind = ("u8", "f32")[j % 2]
+        x = O.gen(kind, (300, 530), 7100 + j)
+        yd = torch.empty((300, 530), dtype=torch.uint32 if kind == "u8" else torch.float32, device=dev)
+        imgs.append((kind, x, torch.from_numpy(x).to(dev), yd))
+    plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+    plan.run()
+    torch.cuda.synchronize()
+    check_batch(O, imgs)
+    plan.check_faults()
