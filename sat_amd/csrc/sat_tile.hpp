@@ -113,7 +113,9 @@ struct TileOp {
   static_assert(TW % (64 * NVW) == 0, "vertical lanes");
 
   struct SmemCarry {
-    A ct[DO_COL ? NW : 1][TW];  // per-wave column sums
+    // per-wave column sums; for 8-byte values also the store stage (16-byte
+    // aligned for its 16-byte LDS accesses)
+    alignas(sizeof(A) == 8 ? 16 : alignof(A)) A ct[DO_COL ? NW : 1][TW];
     A cab[DO_COL ? TW : 1];     // column sums above the tile
     A rs[DO_ROW ? TH : 1];      // row sums of the tile
     A rc[DO_ROW ? TH : 1];      // row carry of each tile row
@@ -511,6 +513,17 @@ struct TileOp {
     if constexpr (!TOUT) {
       TDst* __restrict__ out = (TDst*)g.out;
       const bool fast = interior(g, k);
+      // 8-byte outputs of interior tiles go through the wave's row of sm.ct
+      // (free once every wave is past col_add), so that each store
+      // instruction writes 1 KB of whole lines (lane l: columns 2l, 2l+1, then
+      // 128 + 2l, 129 + 2l) instead of 16 of every 32 bytes, and can be
+      // nontemporal like the 4-byte outputs' stores: f32 -> f64 96 x 1024^2
+      // 286 -> 263 us, f64 4096^2 -2 %, f64 8192^2 equal, the 512-image
+      // batch with f64 outputs +0.7 %
+      constexpr bool STAGE8 = sizeof(TDst) == 8 && DO_COL;
+      if constexpr (STAGE8) {
+        if (fast) __syncthreads();
+      }
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int li = w * RPT + r;
@@ -523,7 +536,23 @@ struct TileOp {
           if constexpr (DO_ROW) s += rp[r];
           y[REV ? 3 - c : c] = s;
         }
-        if (fast) {
+        if (STAGE8 && fast) {
+          typedef TDst d2 __attribute__((ext_vector_type(2)));
+          A* st = &sm.ct[w][0];
+          const int qc = REV ? 63 - lane : lane;  // physical chunk of this lane's 4 columns
+          *(d2*)&st[4 * qc] = d2{cvt_out<TDst>(y[0]), cvt_out<TDst>(y[1])};
+          *(d2*)&st[4 * qc + 2] = d2{cvt_out<TDst>(y[2]), cvt_out<TDst>(y[3])};
+          // a wave's LDS accesses complete in order; the barriers keep the
+          // compiler from moving the reads above the writes, or the next
+          // row's writes above these reads
+          __builtin_amdgcn_wave_barrier();
+          const d2 a = *(const d2*)&st[2 * lane];
+          const d2 b = *(const d2*)&st[128 + 2 * lane];
+          __builtin_amdgcn_wave_barrier();
+          TDst* orow = out + prow * g.ldo + px0;
+          __builtin_nontemporal_store(a, (d2*)(orow + 2 * lane));
+          __builtin_nontemporal_store(b, (d2*)(orow + 128 + 2 * lane));
+        } else if (fast) {
           typename VecG<TDst>::type q;
           q.x = cvt_out<TDst>(y[0]);
           q.y = cvt_out<TDst>(y[1]);
@@ -532,9 +561,10 @@ struct TileOp {
           TDst* o0 = out + (py0 + (REV ? TH - 1 - w * RPT : w * RPT)) * g.ldo + pcol;
           // streaming output: nontemporal stores keep it from displacing
           // the carry words and the input in the caches. An 8-byte output
-          // row chunk is two 16-byte stores per lane, each covering half of
-          // every 64-byte line: plain stores, which L2 can merge, measured
-          // 3-5 % faster there (f64 4096^2, f32 -> f64 96 x 1024^2)
+          // row chunk of the row pass (no stage) is two 16-byte stores per
+          // lane, each covering half of every 64-byte line: plain stores,
+          // which L2 can merge, measured 3-5 % faster than nontemporal ones
+          // there (on the 2-D SAT before it had the stage)
           typename VecG<TDst>::type* oq = (typename VecG<TDst>::type*)(o0 + r * (REV ? -g.ldo : g.ldo));
           if constexpr (sizeof(TDst) == 8) *oq = q;
           else __builtin_nontemporal_store(q, oq);
