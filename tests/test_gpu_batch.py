@@ -304,3 +304,39 @@ def test_batch_equal_images_one_more_than_xcds(O, dev, sa):
     torch.cuda.synchronize()
     check_batch(O, imgs)
     plan.check_faults()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_batch_random_compositions(O, dev, sa, seed):
+    """Seeded batches of k large images (sides 500..1400) and n small ones
+    (sides 1..220), k in {0, 1, 2, 3, 9}, n in {0, 5, 30}, mixed u8 -> u32 /
+    f32, each launch form: whatever mix of images laid over all XCDs and
+    images dealt whole the planner picks, every image against the oracle."""
+    rng = np.random.default_rng(9000 + seed)
+    L = sa.lib()
+    before = L.sat_batch_persistent_get()
+    try:
+        for case in range(4):
+            k = int(rng.choice([0, 1, 2, 3, 9]))
+            n = int(rng.choice([0, 5, 30])) if k else int(rng.choice([5, 30]))
+            persistent = int(rng.integers(0, 2))
+            say(f"[case] seed {seed} case {case}: {k} large + {n} small images, persistent={persistent}")
+            assert L.sat_batch_persistent_set(persistent) == 0
+            imgs = []
+            for j in range(k):
+                h, w = (int(v) for v in rng.integers(500, 1401, 2))
+                kind = ("u8", "f32")[int(rng.integers(0, 2))]
+                x = O.gen(kind, (h, w), 90000 + 1000 * seed + 100 * case + j)
+                yd = torch.empty((h, w), dtype=torch.uint32 if kind == "u8" else torch.float32, device=dev)
+                imgs.append((kind, x, torch.from_numpy(x).to(dev), yd))
+            if n:
+                imgs += make_batch(O, dev, n, 1, 220, 500 + 10 * seed + case)
+            order = rng.permutation(len(imgs))
+            imgs = [imgs[i] for i in order]
+            plan = sa.BatchPlan([(xd, yd) for _, _, xd, yd in imgs])
+            plan.run()
+            torch.cuda.synchronize()
+            check_batch(O, imgs)
+            plan.check_faults()
+    finally:
+        L.sat_batch_persistent_set(before)
