@@ -132,7 +132,12 @@ __global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW))
 // the one-item kernel: at its configuration's waves per SIMD it spilled 60-135
 // registers and took 1.75 ms on the 512-image workload; with a register budget
 // of 2 waves per SIMD it spills none and takes 1.14 ms (the one-item form:
-// 0.91 ms).
+// 0.91 ms). The f32 form of configuration 1 takes 212 VGPRs in the loop
+// against at most 128 for one item: values that depend only on the lane
+// (LDS and row offsets of every dtype pair's inlined operator) are hoisted
+// out of the loop and stay live across items. A call per item instead of the
+// inlined operators would save and restore the callee-saved registers it
+// uses through scratch memory on every tile.
 template <int OUTB, int CFG>
 __global__ void __launch_bounds__((64 * BatchCfg<OUTB, CFG>::NW))
     __attribute__((amdgpu_waves_per_eu(BatchCfg<OUTB, CFG>::WPE < 2 ? BatchCfg<OUTB, CFG>::WPE : 2)))
