@@ -343,3 +343,29 @@ def test_random_pass_shapes(O, dev, sa):
                 ref, sc = O.transpose(ref), O.transpose(sc)
             u = 2.0 ** -24 if tout == "f32" else 2.0 ** -53
             assert np.all(np.abs(g.astype(np.float64) - ref) <= 32 * u * sc + 1e-300), f"{name} {h}x{w}"
+
+
+@pytest.mark.parametrize("tin", ["f32", "f64", "u8"])
+def test_f64_outputs_odd_pitch_and_offset(O, dev, sa, tin):
+    """8-byte outputs at an odd row pitch and a base one element past a
+    16-byte boundary (rows only 8-byte aligned), on shapes with interior
+    tiles, every algorithm: the results match the oracle and the padding
+    around the image is never written."""
+    import ctypes as C
+
+    L = sa.lib()
+    st = torch.cuda.current_stream().cuda_stream
+    h, w, pad = 300, 777, 6
+    pitch = w + pad + 1  # odd
+    x = O.gen(KIND[tin], (h, w), 7300 + len(tin))
+    xd = to_dev(x, dev)
+    for alg in range(4):
+        say(f"[case] {tin}->f64 alg={alg} {h}x{w} out pitch {pitch} offset 1")
+        buf = torch.full((h, pitch), float("nan"), dtype=torch.float64, device=dev)
+        code = L.sat_integral(alg, 0, sa.sat_dtype(TD[tin]), sa.sat_dtype(torch.float64), C.c_void_p(xd.data_ptr()),
+                              w, C.c_void_p(buf.data_ptr() + 8), pitch, w, h, None, 0, C.c_void_p(st))
+        assert code == 0, sa.STATUS.get(code, code)
+        torch.cuda.synchronize()
+        got = buf.cpu().numpy()
+        assert np.isnan(got[:, 0]).all() and np.isnan(got[:, w + 1:]).all(), "padding written"
+        assert_sat_close(O, x, np.ascontiguousarray(got[:, 1:w + 1]), "f64")
